@@ -1,0 +1,165 @@
+"""ctypes mirror of include/cil_hip.h and the loader of the in-tree gfx950 library.
+
+The structs below follow the C declarations field by field; `check_layout()` compares
+their sizes with the library's own `cil_hip_sizeof`.  Loading fails loudly when the HIP
+library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+MAX_SPECIES = 4
+MAX_ELEMS = 2
+MAX_TYPES = MAX_SPECIES * MAX_ELEMS
+MAX_WALLS = 4
+ABI_VERSION = 1
+
+# cil_status
+OK, E_INVALID, E_NO_DEVICE, E_CAPACITY, E_OUTSIDE, E_NONFINITE, E_HIP, E_BOND = 0, -1, -2, -3, -4, -5, -6, -7
+# cil_law
+LAW_NONE, LAW_LJ, LAW_SC, LAW_HZ, LAW_EA = 0, 1, 2, 3, 4
+# cil_spring
+SPRING_FENE, SPRING_HARMONIC = 0, 1
+# cil_noise
+NOISE_NONE, NOISE_CELL_WIENER, NOISE_FDT = 0, 1, 2
+# cil_rng
+RNG_COUNTER, RNG_EXTERNAL = 0, 1
+# cil_division
+DIV_NONE, DIV_THETA_CYCLE, DIV_LINEAR_GROWTH = 0, 1, 2
+# cil_precision
+PREC_FP64, PREC_FP32 = 0, 1
+# CellState (Cell.cuh:26-30)
+CS_DIVIDING, CS_CRAWLING, CS_UNDEFINED = 0, 1, 2
+
+D2 = C.c_double * MAX_ELEMS
+
+
+class Gaussian(C.Structure):
+    _fields_ = [("mu", C.c_double), ("sigma", C.c_double)]
+
+
+class Species(C.Structure):
+    _fields_ = [
+        ("nelem", C.c_int),
+        ("sigma", D2), ("zeta", D2), ("zeta_p", D2), ("zeta_p_beta", D2), ("motility", D2),
+        ("eps_core", D2), ("eps_well", D2), ("sig_well", D2),
+        ("cross_adhesion", C.c_double),
+        ("spring", C.c_int),
+        ("kappa", C.c_double), ("kappa_push", C.c_double), ("spring_ratio", C.c_double),
+        ("r2max", C.c_double), ("div_r2max", C.c_double),
+        ("rss", C.c_double),
+        ("cil", C.c_int), ("quad_drag", C.c_int),
+        ("k_harm", C.c_double),
+        ("d_noise", C.c_double), ("d_noise_radial", C.c_double), ("d_noise_angular", C.c_double),
+        ("kT", C.c_double),
+        ("cycle_p0", C.c_double), ("cycle_tau_r", C.c_double), ("cycle_tau_v", C.c_double),
+        ("cycle_tau_p", C.c_double), ("cycle_v_s", C.c_double), ("cycle_v_1", C.c_double),
+        ("cycle_beta_vs", C.c_double),
+        ("cycle_rmaxed_inv", C.c_double), ("cycle_rho_ratio", C.c_double), ("cycle_p_shift", C.c_double),
+        ("cycle_p_shift_tau_inv", C.c_double), ("apop_pressure", C.c_double),
+        ("division_tau", Gaussian), ("migration_tau", Gaussian),
+        ("growth_rate", C.c_double), ("l_div", C.c_double), ("l_birth", C.c_double), ("sigma_jitter", C.c_double),
+    ]
+
+
+class Wall(C.Structure):
+    _fields_ = [("kind", C.c_int), ("axis", C.c_int), ("x0", C.c_double), ("rcut", C.c_double),
+                ("epsilon", C.c_double), ("positive", C.c_int)]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int), ("dim", C.c_int), ("nspecies", C.c_int),
+        ("species", Species * MAX_SPECIES),
+        ("law", (C.c_int * MAX_TYPES) * MAX_TYPES),
+        ("rcut2", (C.c_double * MAX_TYPES) * MAX_TYPES),
+        ("force_shifted", C.c_int),
+        ("dt", C.c_double),
+        ("noise", C.c_int), ("active_noise", C.c_int), ("rng", C.c_int),
+        ("seed", C.c_uint64),
+        ("division", C.c_int),
+        ("osmotic_pi", C.c_double),
+        ("nwalls", C.c_int), ("walls", Wall * MAX_WALLS),
+        ("drain", C.c_int), ("precision", C.c_int), ("pressure", C.c_int),
+    ]
+
+
+class Box(C.Structure):
+    _fields_ = [("length", C.c_double * 3), ("periodic", C.c_int * 3)]
+
+
+PD = C.POINTER(C.c_double)
+PI = C.POINTER(C.c_int)
+
+
+class SpeciesView(C.Structure):
+    _fields_ = [
+        ("ncells", C.c_int), ("capacity", C.c_int),
+        ("pos", PD), ("frc", PD), ("vel", PD), ("sigma", PD), ("sigma_t", PD), ("pressure", PD),
+        ("state", PI), ("t0", PI), ("tswitch", PI), ("tpressure", PI),
+        ("theta", PD), ("rho", PD), ("ang_vel", PD), ("rest_len", PD),
+    ]
+
+
+VIEW_DOUBLE_FIELDS = ("pos", "frc", "vel", "sigma", "sigma_t", "pressure", "theta", "rho", "ang_vel", "rest_len")
+VIEW_INT_FIELDS = ("state", "t0", "tswitch", "tpressure")
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcil_hip.so")
+_lib = None
+
+
+class CilError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cil_hip error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load the in-tree gfx950 library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.cil_hip_create.argtypes = [C.POINTER(Params), C.POINTER(Box), PI, C.POINTER(vp)]
+    L.cil_hip_destroy.argtypes = [vp]
+    L.cil_hip_upload.argtypes = [vp, C.POINTER(SpeciesView), C.c_int]
+    L.cil_hip_step.argtypes = [vp, C.c_int, PI]
+    L.cil_hip_download.argtypes = [vp, C.POINTER(SpeciesView), C.c_int, PI]
+    L.cil_hip_forces.argtypes = [vp, PD, PD]
+    L.cil_hip_pairs.argtypes = [vp, C.POINTER(C.c_int32), C.c_size_t, C.POINTER(C.c_size_t)]
+    L.cil_hip_check_positions.argtypes = [vp, PI]
+    L.cil_hip_set_external_noise.argtypes = [vp, PD, C.c_size_t, C.c_int]
+    L.cil_hip_num_cells.argtypes = [vp, PI]
+    L.cil_hip_timings.argtypes = [vp, PD, C.c_int, PI, C.c_char_p, C.c_size_t]
+    L.cil_hip_bin_info.argtypes = [vp, PI, PD]
+    L.cil_hip_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    L.cil_hip_set_timing.argtypes = [vp, C.c_int]
+    L.cil_hip_last_error.argtypes = [vp]
+    L.cil_hip_last_error.restype = C.c_char_p
+    L.cil_hip_version.restype = C.c_char_p
+    L.cil_hip_sizeof.argtypes = [C.c_int]
+    L.cil_hip_sizeof.restype = C.c_size_t
+    _lib = L
+    return L
+
+
+def check_layout(L=None):
+    L = L or lib()
+    want = {0: Params, 1: Species, 2: SpeciesView, 3: Box}
+    for k, cls in want.items():
+        got = L.cil_hip_sizeof(k)
+        if got != C.sizeof(cls):
+            raise RuntimeError(f"ABI layout mismatch for {cls.__name__}: C {got} vs ctypes {C.sizeof(cls)}")
+    return True
+
+
+EXPORTS = (
+    "cil_hip_create", "cil_hip_destroy", "cil_hip_upload", "cil_hip_step", "cil_hip_download",
+    "cil_hip_forces", "cil_hip_pairs", "cil_hip_check_positions", "cil_hip_set_external_noise",
+    "cil_hip_num_cells", "cil_hip_timings", "cil_hip_bin_info", "cil_hip_set_tuning", "cil_hip_set_timing",
+    "cil_hip_sizeof", "cil_hip_last_error", "cil_hip_version",
+)

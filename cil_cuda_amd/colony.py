@@ -1,0 +1,199 @@
+"""Host-side mirror of the reference's colony interfaces over the C ABI.
+
+`Colony` holds per-species arrays in the reference's layout (CellColony, Colony.hpp:81-113:
+pos(cell, elem, d), sigmaCore(cell, elem), per-cell state), and `HipMD` is the drop-in
+Euler step (`mdStep = EulerHIP`, MD.cuh:450-452) running on the GPU through
+include/cil_hip.h.  Everything here is plumbing; the step itself runs in the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+class Colony:
+    """Per-species cell arrays (numpy), reference layout, with a fixed capacity."""
+
+    DOUBLE_CELL = ("sigma_t", "pressure", "theta", "rho", "ang_vel", "rest_len")
+    INT_CELL = ("state", "t0", "tswitch", "tpressure")
+
+    def __init__(self, dim: int, nelem: int, capacities, ncells=None):
+        self.dim = dim
+        self.nelem = nelem
+        self.species = []
+        for k, cap in enumerate(capacities):
+            cap = int(cap)
+            sp = {
+                "ncells": 0 if ncells is None else int(ncells[k]),
+                "capacity": cap,
+                "pos": np.zeros((cap, nelem, dim)),
+                "frc": np.zeros((cap, nelem, dim)),
+                "vel": np.zeros((cap, nelem, dim)),
+                "sigma": np.ones((cap, nelem)),
+            }
+            for f in self.DOUBLE_CELL:
+                sp[f] = np.zeros(cap)
+            for f in self.INT_CELL:
+                sp[f] = np.zeros(cap, dtype=np.int32)
+            sp["state"][:] = abi.CS_UNDEFINED
+            sp["tswitch"][:] = -1
+            self.species.append(sp)
+
+    @property
+    def nspec(self):
+        return len(self.species)
+
+    def ncells(self):
+        return [sp["ncells"] for sp in self.species]
+
+    def total_cells(self):
+        return sum(self.ncells())
+
+    def copy(self):
+        c = Colony(self.dim, self.nelem, [sp["capacity"] for sp in self.species])
+        for a, b in zip(c.species, self.species):
+            for k, v in b.items():
+                a[k] = v.copy() if isinstance(v, np.ndarray) else v
+        return c
+
+    def views(self, with_frc=True):
+        arr = (abi.SpeciesView * self.nspec)()
+        for k, sp in enumerate(self.species):
+            v = arr[k]
+            v.ncells = sp["ncells"]
+            v.capacity = sp["capacity"]
+            for f in abi.VIEW_DOUBLE_FIELDS:
+                if f == "frc" and not with_frc:
+                    setattr(v, f, abi.PD())
+                    continue
+                a = sp[f]
+                assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+                setattr(v, f, a.ctypes.data_as(abi.PD))
+            for f in abi.VIEW_INT_FIELDS:
+                a = sp[f]
+                assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+                setattr(v, f, a.ctypes.data_as(abi.PI))
+        return arr
+
+    def pull_counts(self, views):
+        for k, sp in enumerate(self.species):
+            sp["ncells"] = int(views[k].ncells)
+
+    def flat(self, field):
+        """Concatenate the active part of a per-element field over species (UID order)."""
+        return np.concatenate([sp[field][: sp["ncells"]] for sp in self.species], axis=0)
+
+
+def _check(L, ctx, rc, what):
+    if rc != 0:
+        msg = L.cil_hip_last_error(ctx).decode() if ctx else ""
+        raise abi.CilError(rc, f"{what}: {msg}")
+
+
+class HipMD:
+    """GPU Euler step behind the C ABI (the drop-in for the reference's mdStep)."""
+
+    def __init__(self, params: abi.Params, box: abi.Box, capacity):
+        self.L = abi.lib()
+        abi.check_layout(self.L)
+        caps = (C.c_int * abi.MAX_SPECIES)(*([int(c) for c in capacity] + [0] * (abi.MAX_SPECIES - len(capacity))))
+        self.ctx = C.c_void_p()
+        rc = self.L.cil_hip_create(C.byref(params), C.byref(box), caps, C.byref(self.ctx))
+        if rc != 0:
+            raise abi.CilError(rc, "cil_hip_create failed")
+        self.params = params
+        self.nspec = params.nspecies
+        self.dim = params.dim
+        self.nelem = params.species[0].nelem
+
+    def close(self):
+        if self.ctx:
+            self.L.cil_hip_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, colony: Colony, with_frc=False):
+        v = colony.views(with_frc=with_frc)
+        _check(self.L, self.ctx, self.L.cil_hip_upload(self.ctx, v, colony.nspec), "upload")
+
+    def download(self, colony: Colony):
+        v = colony.views()
+        req = (C.c_int * abi.MAX_SPECIES)()
+        _check(self.L, self.ctx, self.L.cil_hip_download(self.ctx, v, colony.nspec, req), "download")
+        colony.pull_counts(v)
+        return colony
+
+    def forces(self, want=True):
+        """calculForcesPressure at the current state; returns (frc[Np,D], pressure[Nc])."""
+        if not want:
+            _check(self.L, self.ctx, self.L.cil_hip_forces(self.ctx, abi.PD(), abi.PD()), "forces")
+            return None
+        nc = self.num_cells()
+        ntot = sum(nc)
+        frc = np.zeros((ntot * self.nelem, self.dim))
+        pr = np.zeros(max(ntot, 1))
+        _check(self.L, self.ctx, self.L.cil_hip_forces(self.ctx, frc.ctypes.data_as(abi.PD), pr.ctypes.data_as(abi.PD)),
+               "forces")
+        return frc, pr[:ntot]
+
+    def step(self, nsteps: int, ts: int) -> int:
+        t = C.c_int(ts)
+        _check(self.L, self.ctx, self.L.cil_hip_step(self.ctx, int(nsteps), C.byref(t)), "step")
+        return t.value
+
+    def pairs(self, cap=None):
+        if cap is None:
+            cap = 1 << 16
+        while True:
+            buf = np.zeros(2 * cap, dtype=np.int32)
+            n = C.c_size_t()
+            _check(self.L, self.ctx,
+                   self.L.cil_hip_pairs(self.ctx, buf.ctypes.data_as(C.POINTER(C.c_int32)), cap, C.byref(n)), "pairs")
+            if n.value <= cap:
+                return buf[: 2 * n.value].reshape(-1, 2)
+            cap = int(n.value)
+
+    def check_positions(self) -> int:
+        r = C.c_int()
+        _check(self.L, self.ctx, self.L.cil_hip_check_positions(self.ctx, C.byref(r)), "check_positions")
+        return r.value
+
+    def set_external_noise(self, normals: np.ndarray):
+        normals = np.ascontiguousarray(normals, dtype=np.float64)
+        nsteps, per = normals.shape
+        _check(self.L, self.ctx,
+               self.L.cil_hip_set_external_noise(self.ctx, normals.ctypes.data_as(abi.PD), per, nsteps),
+               "set_external_noise")
+
+    def num_cells(self):
+        nc = (C.c_int * abi.MAX_SPECIES)()
+        _check(self.L, self.ctx, self.L.cil_hip_num_cells(self.ctx, nc), "num_cells")
+        return [nc[k] for k in range(self.nspec)]
+
+    def bin_info(self):
+        nb = (C.c_int * 3)()
+        w = (C.c_double * 3)()
+        _check(self.L, self.ctx, self.L.cil_hip_bin_info(self.ctx, nb, w), "bin_info")
+        return list(nb), list(w)
+
+    def set_tuning(self, bins_per_cutoff=0, block=0):
+        _check(self.L, self.ctx, self.L.cil_hip_set_tuning(self.ctx, int(bins_per_cutoff), int(block)), "set_tuning")
+
+    def set_timing(self, on=True):
+        _check(self.L, self.ctx, self.L.cil_hip_set_timing(self.ctx, 1 if on else 0), "set_timing")
+
+    def timings(self):
+        us = (C.c_double * 16)()
+        n = C.c_int()
+        names = C.create_string_buffer(256)
+        _check(self.L, self.ctx, self.L.cil_hip_timings(self.ctx, us, 16, C.byref(n), names, 256), "timings")
+        keys = names.value.decode().split(";")
+        return {keys[k]: us[k] for k in range(n.value)}

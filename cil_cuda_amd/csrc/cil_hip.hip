@@ -1,0 +1,825 @@
+// cil_hip.hip — MI355X (gfx950) implementation of the overdamped colony Euler step behind
+// the C ABI of include/cil_hip.h.
+//
+// Step (one Euler(ts) of CIL.cu:1013-1020), kernels in launch order:
+//   k_update     per cell: lazy intra-cell force (CIL.cu:30-100) + sorted pair force ->
+//                fused overdamped update (CIL.cu:902-939); when no division is configured
+//                it also emits the particle's bin key and arrival rank (counting sort pass 1)
+//   [division]   k_div_* flag / prefix-sum / append in parent order (CIL.cu:469-619)
+//   [k_keys]     bin key + rank, when division moved particles
+//   [k_noise]    Wiener deviates per cell (CIL.cu:793-818), reference noise mode only
+//   k_scan_*     exclusive prefix sum of bin counts
+//   k_scatter    particle -> slot in bin order
+//   k_binsort    one wavefront per bin: bitonic sort of the bin's particle ids in LDS (so
+//                the order is deterministic), gather of the bin-local coordinates
+//   k_pair<T>    one wavefront per bin: full-shell gather over the stencil bins, canonical
+//                (lower-UID) orientation per pair so asymmetric terms match the reference
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+using namespace cil;
+
+#define WAVE 64
+
+// ======================================================================================
+// Bin keys
+// ======================================================================================
+__global__ void k_keys(const StepParams* __restrict__ Pp, Dev S) {
+  const StepParams& P = *Pp;
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= S.P) return;
+  int s, c;
+  int cslot = p / P.nE;
+  if (!cell_active(P, S.ncells, cslot, &s, &c)) {
+    S.key[p] = -1;
+    return;
+  }
+  double x[3];
+  bool finite = true;
+  for (int d = 0; d < P.box.dim; ++d) {
+    x[d] = S.pos[(size_t)d * S.P + p];
+    finite = finite && isfinite(x[d]);
+  }
+  if (!finite) {
+    atomicOr(S.err, (unsigned)ERR_NONFINITE);
+    for (int d = 0; d < P.box.dim; ++d) x[d] = 0.0;
+  }
+  int b = bin_of(P, x);
+  S.key[p] = b;
+  S.rank[p] = atomicAdd(&S.count[b], 1);
+}
+
+// ======================================================================================
+// Exclusive prefix sum (3 small kernels): 4096 items per block of 256 threads.
+// ======================================================================================
+#define SCAN_ITEMS 16
+#define SCAN_BLOCK 256
+#define SCAN_TILE (SCAN_ITEMS * SCAN_BLOCK)
+
+__device__ inline int block_exclusive_scan(int v, int* lds, int* total) {
+  int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    int y = __shfl_up(x, o, WAVE);
+    if (lane >= o) x += y;
+  }
+  if (lane == WAVE - 1) lds[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int k = 0; k < (int)(blockDim.x / WAVE); ++k) {
+      int t = lds[k];
+      lds[k] = acc;
+      acc += t;
+    }
+    lds[WAVE] = acc;
+  }
+  __syncthreads();
+  int res = x - v + lds[wid];
+  *total = lds[WAVE];
+  __syncthreads();
+  return res;
+}
+
+__global__ void k_scan_local(const int* __restrict__ in, int* __restrict__ out, int* __restrict__ bsum, int n) {
+  __shared__ int lds[WAVE + 1];
+  int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  int v[SCAN_ITEMS];
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    sum += v[k];
+  }
+  int total;
+  int ex = block_exclusive_scan(sum, lds, &total);
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    if (base + k < n) out[base + k] = ex;
+    ex += v[k];
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ void k_scan_top(int* __restrict__ bsum, int nb, int* __restrict__ total_out) {
+  __shared__ int lds[WAVE + 1];
+  int carry = 0;
+  for (int base = 0; base < nb; base += blockDim.x) {
+    int i = base + threadIdx.x;
+    int v = i < nb ? bsum[i] : 0;
+    int total;
+    int ex = block_exclusive_scan(v, lds, &total);
+    if (i < nb) bsum[i] = ex + carry;
+    carry += total;
+  }
+  if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ bsum, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += bsum[i / SCAN_TILE];
+}
+
+// ======================================================================================
+// Scatter and per-bin sort + gather
+// ======================================================================================
+__global__ void k_scatter(Dev S) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= S.P) return;
+  int b = S.key[p];
+  if (b < 0) return;
+  S.idx[S.start[b] + S.rank[p]] = p;
+}
+
+// Orders this wave's LDS accesses across lanes (wave-scope fence + wave barrier).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+#define BINSORT_MAX 1024
+#define BINSORT_WAVES 4
+
+template <typename T>
+__device__ inline void write_sorted(const StepParams& P, const Dev& S, int b, int k, int p);
+
+template <>
+__device__ inline void write_sorted<float>(const StepParams& P, const Dev& S, int b, int k, int p) {
+  int bc[3];
+  if (P.box.dim == 3) {
+    bc[2] = b % P.ns[2];
+    int t = b / P.ns[2];
+    bc[1] = t % P.ns[1];
+    bc[0] = t / P.ns[1];
+  } else {
+    bc[1] = b % P.ns[1];
+    bc[0] = b / P.ns[1];
+    bc[2] = 0;
+  }
+  float4 r;
+  r.x = (float)(S.pos[p] - bc[0] * P.w[0]);
+  r.y = (float)(S.pos[(size_t)S.P + p] - bc[1] * P.w[1]);
+  r.z = P.box.dim == 3 ? (float)(S.pos[2 * (size_t)S.P + p] - bc[2] * P.w[2]) : 0.0f;
+  r.w = (float)S.sigma[p];
+  ((float4*)S.spos)[k] = r;
+}
+template <>
+__device__ inline void write_sorted<double>(const StepParams& P, const Dev& S, int b, int k, int p) {
+  double4 r;
+  r.x = S.pos[p];
+  r.y = S.pos[(size_t)S.P + p];
+  r.z = P.box.dim == 3 ? S.pos[2 * (size_t)S.P + p] : 0.0;
+  r.w = S.sigma[p];
+  ((double4*)S.spos)[k] = r;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(WAVE* BINSORT_WAVES) k_binsort(const StepParams* __restrict__ Pp, Dev S) {
+  const StepParams& P = *Pp;
+  __shared__ int buf[BINSORT_WAVES][BINSORT_MAX];
+  int wid = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+  int b = blockIdx.x * BINSORT_WAVES + wid;
+  if (b >= P.nbins) return;
+  int s0 = S.start[b], n = S.start[b + 1] - s0;
+  if (lane == 0) S.count[b] = 0;  // reset for the next step's counting pass
+  if (n == 0) return;
+  int* L = buf[wid];
+  if (n <= WAVE) {
+    // register bitonic sort across the wave
+    int v = lane < n ? S.idx[s0 + lane] : 0x7fffffff;
+#pragma unroll
+    for (int k = 2; k <= WAVE; k <<= 1) {
+#pragma unroll
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        int o = __shfl_xor(v, j, WAVE);
+        bool up = ((lane & k) == 0);
+        bool lower = ((lane & j) == 0);
+        int mn = v < o ? v : o, mx = v < o ? o : v;
+        v = (lower == up) ? mn : mx;
+      }
+    }
+    if (lane < n) {
+      S.idx[s0 + lane] = v;
+      int p = v;
+      int cslot = p / P.nE, e = p - cslot * P.nE;
+      int sp = species_of_cell(P, cslot);
+      S.smeta[s0 + lane] = p * 8 + (sp * P.nE + e);
+      S.inv[p] = s0 + lane;
+      write_sorted<T>(P, S, b, s0 + lane, p);
+    }
+    return;
+  }
+  if (n > BINSORT_MAX) {
+    if (lane == 0) atomicOr(S.err, (unsigned)ERR_BIN_OVERFLOW);
+    return;
+  }
+  // One wave sorts its own LDS region; only wave-local ordering is needed between stages
+  // (no workgroup barrier: the other waves of the block take other paths).
+  int np2 = 1;
+  while (np2 < n) np2 <<= 1;
+  for (int i = lane; i < np2; i += WAVE) L[i] = i < n ? S.idx[s0 + i] : 0x7fffffff;
+  wave_lds_sync();
+  for (int k = 2; k <= np2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = lane; i < np2; i += WAVE) {
+        int ixj = i ^ j;
+        if (ixj > i) {
+          int a = L[i], c = L[ixj];
+          bool up = ((i & k) == 0);
+          if ((a > c) == up) {
+            L[i] = c;
+            L[ixj] = a;
+          }
+        }
+      }
+      wave_lds_sync();
+    }
+  }
+  for (int i = lane; i < n; i += WAVE) {
+    int p = L[i];
+    S.idx[s0 + i] = p;
+    int cslot = p / P.nE, e = p - cslot * P.nE;
+    int sp = species_of_cell(P, cslot);
+    S.smeta[s0 + i] = p * 8 + (sp * P.nE + e);
+    S.inv[p] = s0 + i;
+    write_sorted<T>(P, S, b, s0 + i, p);
+  }
+}
+
+// ======================================================================================
+// Pair forces: one wavefront per bin, lanes = particles of the bin, uniform loop over
+// the stencil bins' particles (neighbour data is wave-uniform -> scalar/broadcast loads).
+// ======================================================================================
+__device__ inline void bin_coords(const StepParams& P, int b, int* bc) {
+  if (P.box.dim == 3) {
+    bc[2] = b % P.ns[2];
+    int t = b / P.ns[2];
+    bc[1] = t % P.ns[1];
+    bc[0] = t / P.ns[1];
+  } else {
+    bc[1] = b % P.ns[1];
+    bc[0] = b / P.ns[1];
+    bc[2] = 0;
+  }
+}
+
+// FP64 pair evaluation from p's side with the reference's operation order.
+// d = minimg(x_q - x_p); returns true when the pair interacts, adds to F and pr.
+__device__ inline void pair64(const StepParams& P, int pidx_p, int tp, int spec_p, double sp_, int pidx_q, int tq,
+                              int spec_q, double sq_, double d[3], double r2, uint32_t ts, double F[3], double* pr) {
+  const int D = P.box.dim;
+  bool p_is_i = pidx_p < pidx_q;
+  int ti = p_is_i ? tp : tq, tj = p_is_i ? tq : tp;
+  double si = p_is_i ? sp_ : sq_, sj = p_is_i ? sq_ : sp_;
+  const LawEntry& E = P.law[ti][tj];
+  double f0;
+  bool contact = false;
+  double temp = 0.0;
+  if (E.law == CIL_LAW_LJ) {
+    double sij2 = sq(0.5 * (si + sj));
+    double rc2 = sij2 * E.rcut2;
+    if (!(r2 <= rc2)) return;
+    double f = lj_f0(sij2, r2) - (P.force_shifted ? lj_f0(sij2, rc2) * sqrt(rc2 / r2) : 0.0);
+    f0 = f * E.eps;
+  } else if (E.law == CIL_LAW_SC) {
+    double sigCore = 0.5 * (si + sj);
+    double rc2 = E.rcut2 * sq(sigCore + E.xi);
+    if (!(r2 > E.rcut2 * sq(sigCore - E.xi) && r2 <= rc2)) return;
+    f0 = sc_f0(sigCore, r2, E.xi, E.e0, E.ew) -
+         (P.force_shifted ? sc_f0(sigCore, rc2, E.xi, E.e0, E.ew) * sqrt(rc2 / r2) : 0.0);
+  } else if (E.law == CIL_LAW_HZ || E.law == CIL_LAW_EA) {
+    double r = sqrt(r2);
+    if (!(r <= (si + sj) / 2)) return;
+    contact = true;
+    if (D == 2 && d[0] == 0.0 && d[1] == 0.0) {  // MD.cu:321-331, direction for canonical i
+      double u[2];
+      int ci = (p_is_i ? pidx_p : pidx_q) / P.nE, cj = (p_is_i ? pidx_q : pidx_p) / P.nE;
+      int ei = (p_is_i ? pidx_p : pidx_q) - ci * P.nE, ej = (p_is_i ? pidx_q : pidx_p) - cj * P.nE;
+      int si_ = species_of_cell(P, ci), sj_ = species_of_cell(P, cj);
+      uint32_t ki = (cell_key(si_, ci - P.base[si_]) << 1) | ei, kj = (cell_key(sj_, cj - P.base[sj_]) << 1) | ej;
+      philox_uniform2(P.seed, ki, ts, ST_PAIR, kj, u);
+      double alpha = u[0] * kTwoPi;
+      double rij0 = sin(alpha), rij1 = cos(alpha);
+      d[0] = p_is_i ? rij0 : -rij0;
+      d[1] = p_is_i ? rij1 : -rij1;
+    }
+    double hh = (si + sj) / 2 - r;
+    double h = hh >= 0.0 ? hh : 0.0;
+    if (E.law == CIL_LAW_HZ) {
+      double Ri = si / 2, Rj = sj / 2;
+      f0 = -(pow(h, 1.5) / (E.hzk * (sqrt((1 / Ri) + (1 / Rj)))));
+    } else {
+      f0 = -(h * E.ea_ei);
+    }
+    double A = D == 3 ? contact_area(r, si, sj) : contact_length(si, sj, h);
+    f0 += E.ewell_i * A;
+    temp = (A == 0 ? 0 : (4 * kPi * sq(si / 2) / 3));
+    temp = (D == 3 ? temp : A);
+  } else {
+    return;
+  }
+  double s = (f0 < 0.0 || spec_p == spec_q) ? 1.0 : P.cross_adh[spec_p];
+  double fs = s == 1.0 ? f0 : s * f0;
+  for (int k = 0; k < D; ++k) F[k] += fs * d[k];
+  if (contact && P.pressure && temp != 0.0) *pr -= fs / temp;
+}
+
+// Exact FP64 cut-off decision for an LJ pair whose FP32 distance is in the guard band.
+__device__ inline bool lj_inside_exact(const StepParams& P, const Dev& S, int pidx_p, int pidx_q, double rcut2) {
+  double xp[3], xq[3], rij[3];
+  bool p_is_i = pidx_p < pidx_q;
+  int a = p_is_i ? pidx_p : pidx_q, b = p_is_i ? pidx_q : pidx_p;
+  for (int d = 0; d < P.box.dim; ++d) {
+    xp[d] = S.pos[(size_t)d * S.P + a];
+    xq[d] = S.pos[(size_t)d * S.P + b];
+  }
+  double r2 = box_dist(P.box, rij, xp, xq);
+  double sij2 = sq(0.5 * (S.sigma[a] + S.sigma[b]));
+  return r2 <= sij2 * rcut2;
+}
+
+__device__ inline void pair32(const StepParams& P, const Dev& S, int pidx_p, int tp, int spec_p, float sp_,
+                              int pidx_q, int tq, int spec_q, float sq_, float d[3], float r2, uint32_t ts, float F[3],
+                              float* pr) {
+  const int D = P.box.dim;
+  bool p_is_i = pidx_p < pidx_q;
+  int ti = p_is_i ? tp : tq, tj = p_is_i ? tq : tp;
+  const LawEntry& E = P.law[ti][tj];
+  float f0;
+  bool contact = false;
+  float temp = 0.0f;
+  if (E.law == CIL_LAW_LJ) {
+    float rcut2 = (float)E.rcut2;
+    float sh = 0.5f * (sp_ + sq_);
+    float sij2 = sh * sh;
+    float rc2 = sij2 * rcut2;
+    if (r2 > rc2 * 1.00001f) return;
+    if (r2 > rc2 * 0.99999f) {
+      if (!lj_inside_exact(P, S, pidx_p, pidx_q, E.rcut2)) return;
+    }
+    float f = lj_f0(sij2, r2) - (P.force_shifted ? lj_f0(sij2, rc2) * sqrtf(rc2 / r2) : 0.0f);
+    f0 = f * (float)E.eps;
+  } else if (E.law == CIL_LAW_SC) {
+    float xi = (float)E.xi, rcut2 = (float)E.rcut2;
+    float sigCore = 0.5f * (sp_ + sq_);
+    float up = sigCore + xi, lo = sigCore - xi;
+    float rc2 = rcut2 * up * up;
+    if (!(r2 > rcut2 * lo * lo && r2 <= rc2)) return;
+    f0 = sc_f0(sigCore, r2, xi, (float)E.e0, (float)E.ew) -
+         (P.force_shifted ? sc_f0(sigCore, rc2, xi, (float)E.e0, (float)E.ew) * sqrtf(rc2 / r2) : 0.0f);
+  } else if (E.law == CIL_LAW_HZ || E.law == CIL_LAW_EA) {
+    float si = p_is_i ? sp_ : sq_, sj = p_is_i ? sq_ : sp_;
+    float r = sqrtf(r2);
+    if (!(r <= (si + sj) * 0.5f)) return;
+    contact = true;
+    if (D == 2 && d[0] == 0.0f && d[1] == 0.0f) {
+      double u[2];
+      int ci = (p_is_i ? pidx_p : pidx_q) / P.nE, cj = (p_is_i ? pidx_q : pidx_p) / P.nE;
+      int ei = (p_is_i ? pidx_p : pidx_q) - ci * P.nE, ej = (p_is_i ? pidx_q : pidx_p) - cj * P.nE;
+      int si_ = species_of_cell(P, ci), sj_ = species_of_cell(P, cj);
+      uint32_t ki = (cell_key(si_, ci - P.base[si_]) << 1) | ei, kj = (cell_key(sj_, cj - P.base[sj_]) << 1) | ej;
+      philox_uniform2(P.seed, ki, ts, ST_PAIR, kj, u);
+      double alpha = u[0] * kTwoPi;
+      float rij0 = (float)sin(alpha), rij1 = (float)cos(alpha);
+      d[0] = p_is_i ? rij0 : -rij0;
+      d[1] = p_is_i ? rij1 : -rij1;
+    }
+    float hh = (si + sj) * 0.5f - r;
+    float h = hh >= 0.0f ? hh : 0.0f;
+    if (E.law == CIL_LAW_HZ) {
+      float Ri = si * 0.5f, Rj = sj * 0.5f;
+      f0 = -(h * sqrtf(h) / ((float)E.hzk * sqrtf(1.0f / Ri + 1.0f / Rj)));
+    } else {
+      f0 = -(h * (float)E.ea_ei);
+    }
+    float A;
+    if (D == 3) {
+      float ra = si * 0.5f, rb = sj * 0.5f;
+      float t = sqrtf(4.f * r2 * ra * ra - (r2 - rb * rb + ra * ra) * (r2 - rb * rb + ra * ra)) / (2.f * r);
+      A = t > 0.f ? (float)kPi * t * t : 0.f;
+    } else {
+      float ra = si * 0.5f, rb = sj * 0.5f;
+      float q = ra - (2.f * rb * h - h * h) / (2.f * (ra + rb - h));
+      float t = 2.f * sqrtf(ra * ra - q * q);
+      A = isnan(t) ? (si <= sj ? si : sj) : t;
+    }
+    f0 += (float)E.ewell_i * A;
+    temp = (A == 0.f ? 0.f : (4.f * (float)kPi * (si * 0.5f) * (si * 0.5f) / 3.f));
+    temp = (D == 3 ? temp : A);
+  } else {
+    return;
+  }
+  float s = (f0 < 0.0f || spec_p == spec_q) ? 1.0f : (float)P.cross_adh[spec_p];
+  float fs = s * f0;
+  for (int k = 0; k < D; ++k) F[k] = fmaf(fs, d[k], F[k]);
+  if (contact && P.pressure && temp != 0.0f) *pr -= fs / temp;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_pair(const StepParams* __restrict__ Pp, Dev S) {
+  const StepParams& P = *Pp;
+  const int D = P.box.dim;
+  int wid = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+  int b = blockIdx.x * (blockDim.x / WAVE) + wid;
+  if (b >= P.nbins) return;
+  b = __builtin_amdgcn_readfirstlane(b);
+  const int s0 = S.start[b], n = S.start[b + 1] - s0;
+  if (n == 0) return;
+  int bc[3];
+  bin_coords(P, b, bc);
+  const uint32_t ts = (uint32_t)*S.ts;
+  const T* spos = (const T*)S.spos;
+  for (int chunk = 0; chunk < n; chunk += WAVE) {
+    int k = s0 + chunk + lane;
+    bool valid = chunk + lane < n;
+    int kk = valid ? k : s0;
+    int meta = S.smeta[kk];
+    int pidx = meta >> 3, tp = meta & 7;
+    int cell_p = pidx / P.nE;
+    int spec_p = species_of_cell(P, cell_p);
+    T xp[4];
+    if constexpr (sizeof(T) == 4) {
+      float4 r = ((const float4*)spos)[kk];
+      xp[0] = r.x; xp[1] = r.y; xp[2] = r.z; xp[3] = r.w;
+    } else {
+      double4 r = ((const double4*)spos)[kk];
+      xp[0] = r.x; xp[1] = r.y; xp[2] = r.z; xp[3] = r.w;
+    }
+    T F[3] = {0, 0, 0};
+    T pr = 0;
+    const int rx = P.reach[0], ry = P.reach[1], rz = D == 3 ? P.reach[2] : 0;
+    for (int ox = -rx; ox <= rx; ++ox) {
+      int nx = bc[0] + ox;
+      T shx = 0;
+      if (P.box.pbc[0]) {
+        if (nx < 0) nx += P.ns[0];
+        else if (nx >= P.ns[0]) nx -= P.ns[0];
+      } else if (nx < 0 || nx >= P.ns[0]) continue;
+      shx = (T)(ox * P.w[0]);
+      for (int oy = -ry; oy <= ry; ++oy) {
+        int ny = bc[1] + oy;
+        T shy = 0;
+        if (P.box.pbc[1]) {
+          if (ny < 0) ny += P.ns[1];
+          else if (ny >= P.ns[1]) ny -= P.ns[1];
+        } else if (ny < 0 || ny >= P.ns[1]) continue;
+        shy = (T)(oy * P.w[1]);
+        for (int oz = -rz; oz <= rz; ++oz) {
+          int nz = 0;
+          T shz = 0;
+          if (D == 3) {
+            nz = bc[2] + oz;
+            if (P.box.pbc[2]) {
+              if (nz < 0) nz += P.ns[2];
+              else if (nz >= P.ns[2]) nz -= P.ns[2];
+            } else if (nz < 0 || nz >= P.ns[2]) continue;
+            shz = (T)(oz * P.w[2]);
+          }
+          int nb = D == 3 ? (nx * P.ns[1] + ny) * P.ns[2] + nz : nx * P.ns[1] + ny;
+          int j0 = S.start[nb], j1 = S.start[nb + 1];
+          for (int j = j0; j < j1; ++j) {
+            int mq = S.smeta[j];
+            int qidx = mq >> 3, tq = mq & 7;
+            int cell_q = qidx / P.nE;
+            if (cell_q == cell_p) continue;
+            int spec_q = species_of_cell(P, cell_q);
+            if constexpr (sizeof(T) == 4) {
+              float4 r = ((const float4*)spos)[j];
+              float d[3];
+              d[0] = (r.x + shx) - xp[0];
+              d[1] = (r.y + shy) - xp[1];
+              d[2] = D == 3 ? (r.z + shz) - xp[2] : 0.0f;
+              if (P.reach[0] == 0 && P.box.pbc[0]) d[0] -= (float)P.box.L[0] * rintf(d[0] * (float)P.box.invL[0]);
+              if (P.reach[1] == 0 && P.box.pbc[1]) d[1] -= (float)P.box.L[1] * rintf(d[1] * (float)P.box.invL[1]);
+              if (D == 3 && P.reach[2] == 0 && P.box.pbc[2]) d[2] -= (float)P.box.L[2] * rintf(d[2] * (float)P.box.invL[2]);
+              float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+              float Fl[3] = {F[0], F[1], F[2]};
+              float prl = pr;
+              pair32(P, S, pidx, tp, spec_p, xp[3], qidx, tq, spec_q, r.w, d, r2, ts, Fl, &prl);
+              F[0] = Fl[0]; F[1] = Fl[1]; F[2] = Fl[2];
+              pr = prl;
+            } else {
+              double4 r = ((const double4*)spos)[j];
+              double xq[3] = {r.x, r.y, r.z};
+              double d[3] = {0, 0, 0};
+              double r2 = 0.0;
+              for (int dd = 0; dd < D; ++dd) {
+                double x = xq[dd] - (double)xp[dd];
+                if (P.box.pbc[dd]) x -= (double)nint(x / P.box.L[dd]) * P.box.L[dd];
+                d[dd] = x;
+                r2 += sq(x);
+              }
+              double Fl[3] = {(double)F[0], (double)F[1], (double)F[2]};
+              double prl = pr;
+              pair64(P, pidx, tp, spec_p, (double)xp[3], qidx, tq, spec_q, r.w, d, r2, ts, Fl, &prl);
+              F[0] = Fl[0]; F[1] = Fl[1]; F[2] = Fl[2];
+              pr = prl;
+            }
+          }
+        }
+      }
+    }
+    if (valid) {
+      T* fp = (T*)S.fpair;
+      for (int dd = 0; dd < D; ++dd) fp[(size_t)dd * S.P + k] = F[dd];
+      ((T*)S.ppair)[k] = pr;
+    }
+  }
+}
+
+// ======================================================================================
+// Noise draws (updateWienerProcess, CIL.cu:793-818), reference noise modes only.
+// ======================================================================================
+__global__ void k_noise(const StepParams* __restrict__ Pp, Dev S) {
+  const StepParams& P = *Pp;
+  int cslot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cslot >= S.C) return;
+  int s, c;
+  if (!cell_active(P, S.ncells, cslot, &s, &c)) return;
+  const int D = P.box.dim;
+  const int ts = *S.ts;
+  const cil_species& sp = P.sp[s];
+  const double* blk = nullptr;
+  long off_passive = 0, off_active = 0;
+  if (P.rng == CIL_RNG_EXTERNAL) {
+    int bi = ts - S.ext_ts0;
+    if (bi < 0 || bi >= S.ext_nsteps) {
+      atomicOr(S.err, (unsigned)ERR_NOISE);
+      return;
+    }
+    blk = S.ext + (long)bi * S.ext_per_step;
+    for (int k = 0; k < P.nspec; ++k) {
+      if (P.noise == CIL_NOISE_CELL_WIENER && P.sp[k].d_noise > 0.0) {
+        if (k < s) off_passive += (long)S.ncells[k] * D;
+        off_active += (long)S.ncells[k] * D;
+      }
+    }
+  }
+  if (P.noise == CIL_NOISE_CELL_WIENER) {
+    for (int d = 0; d < D; ++d) {
+      double w = 0.0;
+      if (sp.d_noise > 0.0) {
+        double z;
+        long at = off_passive + (long)c * D + d;
+        if (blk && at >= S.ext_per_step) {
+          atomicOr(S.err, (unsigned)ERR_NOISE);
+          return;
+        }
+        if (blk) z = blk[at];
+        else {
+          double zz[2];
+          philox_normal2(P.seed, cell_key(s, c), (uint32_t)ts, ST_WIENER, (uint32_t)(d >> 1), zz);
+          z = zz[d & 1];
+        }
+        w = sp.d_noise * z;
+      }
+      S.wiener[(size_t)cslot * 3 + d] = w;
+    }
+  }
+  if (P.active_noise && P.nE == 2) {
+    long ar = off_active, aa = off_active;
+    if (blk) {
+      for (int k = 0; k < P.nspec; ++k) {
+        if (P.sp[k].d_noise_radial > 0.0) {
+          if (k < s) ar += S.ncells[k];
+          aa += S.ncells[k];
+        }
+      }
+      for (int k = 0; k < s; ++k)
+        if (P.sp[k].d_noise_angular > 0.0) aa += S.ncells[k];
+    }
+    double zz[2] = {0, 0};
+    if (blk && (ar + c >= S.ext_per_step || aa + c >= S.ext_per_step)) {
+      atomicOr(S.err, (unsigned)ERR_NOISE);
+      return;
+    }
+    if (!blk) philox_normal2(P.seed, cell_key(s, c), (uint32_t)ts, ST_ACTIVE, 0, zz);
+    double wr = 0, wa = 0;
+    if (sp.d_noise_radial > 0.0) wr = sp.d_noise_radial * (blk ? blk[ar + c] : zz[0]);
+    if (sp.d_noise_angular > 0.0) wa = sp.d_noise_angular * (blk ? blk[aa + c] : zz[1]);
+    S.awiener[(size_t)cslot * 2] = wr;
+    S.awiener[(size_t)cslot * 2 + 1] = wa;
+  }
+}
+
+// ======================================================================================
+// Forces of one cell at the last force evaluation: intra (lazy) + pair (sorted).
+// ======================================================================================
+template <typename T>
+__device__ inline void cell_total_force(const StepParams& P, const Dev& S, int s, int c, int cslot, bool materialized,
+                                        double pos[2][3], double vel[2][3], const double* sig, double F[2][3],
+                                        double* pc, bool* pos1_changed) {
+  const int D = P.box.dim, nE = P.nE;
+  *pos1_changed = false;
+  if (materialized) {
+    for (int e = 0; e < nE; ++e)
+      for (int d = 0; d < D; ++d) F[e][d] = S.frc[(size_t)d * S.P + (size_t)cslot * nE + e];
+    *pc = S.pressure[cslot];
+    return;
+  }
+  intra_cell(P, S, s, c, cslot, *S.ts, pos, vel, sig, F, pos1_changed);
+  const T* fp = (const T*)S.fpair;
+  double pr = 0.0;
+  for (int e = 0; e < nE; ++e) {
+    int k = S.inv[(size_t)cslot * nE + e];
+    for (int d = 0; d < D; ++d) F[e][d] += (double)fp[(size_t)d * S.P + k];
+    if (P.pressure) pr += (double)((const T*)S.ppair)[k];
+  }
+  *pc = pr;
+}
+
+// ======================================================================================
+// Fused update (updateVelocityPositions, CIL.cu:902-939), one thread per cell slot.
+// ======================================================================================
+template <typename T>
+__global__ void __launch_bounds__(256) k_update(const StepParams* __restrict__ Pp, Dev S, int materialized,
+                                                int emit_keys) {
+  const StepParams& P = *Pp;
+  int cslot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cslot >= S.C) return;
+  int s, c;
+  if (!cell_active(P, S.ncells, cslot, &s, &c)) {
+    if (emit_keys)
+      for (int e = 0; e < P.nE; ++e) S.key[(size_t)cslot * P.nE + e] = -1;
+    return;
+  }
+  const int D = P.box.dim, nE = P.nE;
+  const cil_species& sp = P.sp[s];
+  double pos[2][3], vel[2][3], sig[2], F[2][3];
+  for (int e = 0; e < nE; ++e) {
+    size_t p = (size_t)cslot * nE + e;
+    sig[e] = S.sigma[p];
+    for (int d = 0; d < D; ++d) {
+      pos[e][d] = S.pos[(size_t)d * S.P + p];
+      vel[e][d] = S.vel[(size_t)d * S.P + p];
+    }
+  }
+  double pc;
+  bool pos1_changed;
+  cell_total_force<T>(P, S, s, c, cslot, materialized != 0, pos, vel, sig, F, &pc, &pos1_changed);
+  if (!materialized && P.pressure) S.pressure[cslot] = pc;
+  const int tsu = *S.ts;  // step index before the increment (FDT noise key)
+  double area = sigma2area(sig[0]);
+  double pND = pc / sp.cycle_p0;
+  for (int e = 0; e < nE; ++e) {
+    size_t p = (size_t)cslot * nE + e;
+    double pterm = sp.zeta_p[e] == 0.0 ? 0.0 : sp.zeta_p[e] * pow(pND, sp.zeta_p_beta[e]);
+    double temp = P.dt * 1 / ((sp.zeta[e] + pterm) * area);
+    double mu = temp / P.dt;
+    double z[4] = {0, 0, 0, 0};
+    bool fdt = P.noise == CIL_NOISE_FDT && sp.kT > 0.0;
+    if (fdt) {
+      uint32_t id = (cell_key(s, c) << 1) | (uint32_t)e;
+      philox_normal2(P.seed, id, (uint32_t)tsu, ST_FDT, 0, z);
+      philox_normal2(P.seed, id, (uint32_t)tsu, ST_FDT, 1, z + 2);
+    }
+    double amp = fdt ? sqrt(2.0 * sp.kT * temp) : 0.0;
+    double xn[3];
+    for (int d = 0; d < D; ++d) {
+      double x;
+      if (fdt) x = box_update_dr1(P.box, d, pos[e][d], temp * F[e][d] + amp * z[d]);
+      else x = box_update1(P.box, d, pos[e][d], F[e][d], temp);
+      xn[d] = x;
+      S.pos[(size_t)d * S.P + p] = x;
+      S.vel[(size_t)d * S.P + p] = F[e][d] * mu;
+    }
+    if (emit_keys) {
+      bool finite = true;
+      for (int d = 0; d < D; ++d) finite = finite && isfinite(xn[d]);
+      if (!finite) {
+        atomicOr(S.err, (unsigned)ERR_NONFINITE);
+        for (int d = 0; d < D; ++d) xn[d] = 0.0;
+      }
+      int b = bin_of(P, xn);
+      S.key[p] = b;
+      S.rank[p] = atomicAdd(&S.count[b], 1);
+    }
+  }
+}
+
+// Forces at the current state into the canonical FP64 frc array (for download / parity).
+template <typename T>
+__global__ void k_materialize(const StepParams* __restrict__ Pp, Dev S) {
+  const StepParams& P = *Pp;
+  int cslot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cslot >= S.C) return;
+  int s, c;
+  if (!cell_active(P, S.ncells, cslot, &s, &c)) return;
+  const int D = P.box.dim, nE = P.nE;
+  double pos[2][3], vel[2][3], sig[2], F[2][3], pc;
+  for (int e = 0; e < nE; ++e) {
+    size_t p = (size_t)cslot * nE + e;
+    sig[e] = S.sigma[p];
+    for (int d = 0; d < D; ++d) {
+      pos[e][d] = S.pos[(size_t)d * S.P + p];
+      vel[e][d] = S.vel[(size_t)d * S.P + p];
+    }
+  }
+  bool pos1_changed;
+  cell_total_force<T>(P, S, s, c, cslot, false, pos, vel, sig, F, &pc, &pos1_changed);
+  for (int e = 0; e < nE; ++e)
+    for (int d = 0; d < D; ++d) S.frc[(size_t)d * S.P + (size_t)cslot * nE + e] = F[e][d];
+  if (pos1_changed)
+    for (int d = 0; d < D; ++d) S.pos[(size_t)d * S.P + (size_t)cslot * nE + 1] = pos[1][d];
+  if (nE == 1 && P.sp[s].motility[0] != 0.0)
+    for (int d = 0; d < D; ++d) S.vel[(size_t)d * S.P + (size_t)cslot] = vel[0][d];
+  S.pressure[cslot] = P.pressure ? pc : 0.0;
+}
+
+// ======================================================================================
+// Pair-set hook: exact FP64 law predicate over the stencil, emits UID pairs (i<j).
+// ======================================================================================
+__device__ inline int slot_to_uid_particle(const StepParams& P, const int* ncells, int pidx) {
+  int cslot = pidx / P.nE, e = pidx - cslot * P.nE;
+  int s = species_of_cell(P, cslot);
+  int cum = 0;
+  for (int k = 0; k < s; ++k) cum += ncells[k];
+  return (cum + cslot - P.base[s]) * P.nE + e;
+}
+
+__global__ void k_pairs_list(const StepParams* __restrict__ Pp, Dev S, int reach_x, int reach_y, int reach_z,
+                             int* __restrict__ out, unsigned long long* __restrict__ counter, long cap) {
+  const StepParams& P = *Pp;
+  const int D = P.box.dim;
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int total = S.start[P.nbins];
+  if (k >= total) return;
+  int meta = S.smeta[k];
+  int pidx = meta >> 3, tp = meta & 7;
+  int cell_p = pidx / P.nE;
+  double xp[3];
+  for (int d = 0; d < D; ++d) xp[d] = S.pos[(size_t)d * S.P + pidx];
+  double sp_ = S.sigma[pidx];
+  int bc[3];
+  bc[0] = bin_coord(P, 0, xp[0]);
+  bc[1] = bin_coord(P, 1, xp[1]);
+  bc[2] = D == 3 ? bin_coord(P, 2, xp[2]) : 0;
+  int reach[3] = {reach_x, reach_y, D == 3 ? reach_z : 0};
+  for (int ox = -reach[0]; ox <= reach[0]; ++ox)
+    for (int oy = -reach[1]; oy <= reach[1]; ++oy)
+      for (int oz = -reach[2]; oz <= reach[2]; ++oz) {
+        int o[3] = {ox, oy, oz}, nc[3] = {0, 0, 0};
+        bool ok = true;
+        for (int d = 0; d < D; ++d) {
+          int v = bc[d] + o[d];
+          if (P.box.pbc[d]) v = ((v % P.ns[d]) + P.ns[d]) % P.ns[d];
+          else if (v < 0 || v >= P.ns[d]) ok = false;
+          nc[d] = v;
+        }
+        if (!ok) continue;
+        // skip duplicate visits of a bin when the stencil wraps around a small periodic grid
+        bool dup = false;
+        for (int d = 0; d < D && !dup; ++d) {
+          if (P.box.pbc[d] && 2 * reach[d] + 1 > P.ns[d]) {
+            int lo = -reach[d];
+            // canonical offset: smallest offset in [-reach, reach] reaching the same bin
+            for (int t = lo; t < o[d]; ++t)
+              if (((bc[d] + t) % P.ns[d] + P.ns[d]) % P.ns[d] == nc[d]) dup = true;
+          }
+        }
+        if (dup) continue;
+        int nb = D == 3 ? (nc[0] * P.ns[1] + nc[1]) * P.ns[2] + nc[2] : nc[0] * P.ns[1] + nc[1];
+        for (int j = S.start[nb]; j < S.start[nb + 1]; ++j) {
+          int qidx = S.smeta[j] >> 3, tq = S.smeta[j] & 7;
+          if (qidx <= pidx) continue;
+          if (qidx / P.nE == cell_p) continue;
+          int ti = tp, tj = tq;  // p is canonical i (lower slot = lower UID)
+          const LawEntry& E = P.law[ti][tj];
+          if (E.law == CIL_LAW_NONE) continue;
+          double xq[3], rij[3];
+          for (int d = 0; d < D; ++d) xq[d] = S.pos[(size_t)d * S.P + qidx];
+          double r2 = box_dist(P.box, rij, xp, xq);
+          double sq_ = S.sigma[qidx];
+          bool hit = false;
+          if (E.law == CIL_LAW_LJ) {
+            double sij2 = sq(0.5 * (sp_ + sq_));
+            hit = r2 <= sij2 * E.rcut2;
+          } else if (E.law == CIL_LAW_SC) {
+            double sigCore = 0.5 * (sp_ + sq_);
+            hit = r2 > E.rcut2 * sq(sigCore - E.xi) && r2 <= E.rcut2 * sq(sigCore + E.xi);
+          } else {
+            hit = sqrt(r2) <= (sp_ + sq_) / 2;
+          }
+          if (hit) {
+            unsigned long long slot = atomicAdd(counter, 1ull);
+            if ((long)slot < cap) {
+              out[2 * slot] = slot_to_uid_particle(P, S.ncells, pidx);
+              out[2 * slot + 1] = slot_to_uid_particle(P, S.ncells, qidx);
+            }
+          }
+        }
+      }
+}
+
+#include "cil_division.inc"
+#include "cil_context.inc"

@@ -1,0 +1,195 @@
+// kernels.hpp — gfx950 kernels of the colony step.
+//
+// Data layout in HBM (all canonical arrays in cell-slot order, slot = species base + cell id,
+// particle p = slot*nE + elem, matching the reference's pos(cell, elem, d) ordering):
+//   pos/vel/frc : FP64 SoA [D][P]      sigma : FP64 [P]
+//   per-cell    : FP64 / int32 [C]
+// Per-step scratch (rebuilt every step):
+//   key/rank    : int32 [P]  bin index and arrival rank (atomic counting sort)
+//   count/start : int32 [nbins(+1)]
+//   idx         : int32 [P]  sorted slot -> particle, order inside a bin fixed by an
+//                            in-wave bitonic sort so every run is bit-reproducible
+//   spos        : float4 (bin-local x,y,z, sigma) in FP32 mode, double4 (global) in FP64
+//   smeta       : int32 [P]  particle*8 + element type
+//   fpair/ppair : [D][P] / [P] pair forces and pressure partials, in sorted order
+#pragma once
+#include "device.hpp"
+
+namespace cil {
+
+struct Dev {
+  int P, C, D;
+  double* pos;
+  double* vel;
+  double* frc;
+  double* sigma;
+  double* sigma_t;
+  double* pressure;
+  int* state;
+  int* t0;
+  int* tswitch;
+  int* tpressure;
+  double* theta;
+  double* rho;
+  double* ang_vel;
+  double* rest_len;
+  double* wiener;   // [C][3]
+  double* awiener;  // [C][2]
+  int* ncells;      // [CIL_MAX_SPECIES]
+  int* ts;          // [1] step index of the last force evaluation
+  unsigned* err;    // [1]
+  int* key;
+  int* rank;
+  int* count;
+  int* start;       // [nbins+1]
+  int* blocksum;
+  int* idx;
+  int* inv;
+  void* spos;
+  int* smeta;
+  void* fpair;
+  void* ppair;
+  const double* ext;
+  long ext_per_step;
+  int ext_nsteps;
+  int ext_ts0;
+  // division / removal scratch
+  int* dflag;
+  int* dscan;
+  int* dblock;
+  int* nnew;        // [CIL_MAX_SPECIES]
+};
+
+// ---------------------------------------------------------------------------------------
+// Small helpers
+// ---------------------------------------------------------------------------------------
+__device__ inline bool cell_active(const StepParams& P, const int* ncells, int cslot, int* s_out, int* c_out) {
+  int s = species_of_cell(P, cslot);
+  int c = cslot - P.base[s];
+  *s_out = s;
+  *c_out = c;
+  return c < ncells[s];
+}
+
+__device__ inline int bin_coord(const StepParams& P, int d, double x) {
+  int b = (int)floor(x * P.invw[d]);
+  b = b < 0 ? 0 : b;
+  b = b > P.ns[d] - 1 ? P.ns[d] - 1 : b;
+  return b;
+}
+
+__device__ inline int bin_of(const StepParams& P, const double* x) {
+  int b = bin_coord(P, 0, x[0]) * P.ns[1] + bin_coord(P, 1, x[1]);
+  if (P.box.dim == 3) b = b * P.ns[2] + bin_coord(P, 2, x[2]);
+  return b;
+}
+
+// ---------------------------------------------------------------------------------------
+// Intra-cell forces (CIL.cu:30-100): migration (MD.cu:525-644), walls (Wall.hpp:68-79),
+// passive noise.  Writes the full intra force of each element of the cell into F.
+// `vel` may be modified (singleCellForce's zero-velocity kick), pos0/pos1 too (zero bond).
+// ---------------------------------------------------------------------------------------
+__device__ inline void flat_wall(const cil_wall& wl, double sigma, const double* r, double* f) {
+  double dr = (wl.positive ? (r[wl.axis] - wl.x0) : (wl.x0 - r[wl.axis])) / sigma;
+  if (dr <= wl.rcut) {
+    dr = dr >= kWallCutoff ? dr : kWallCutoff;
+    double ir2 = 1.0 / (dr * dr);
+    double f0 = ir2 * ir2 * ir2;
+    f0 = wl.epsilon * (2.0 * f0 - 1.0) * f0 * ir2 / sq(sigma);
+    f0 = wl.positive ? f0 : -f0;
+    f[wl.axis] += f0;
+  }
+}
+
+__device__ inline void on_sphere_counter(uint64_t seed, uint32_t id, uint32_t ts, int D, double* n) {
+  double z[4];
+  philox_normal2(seed, id, ts, ST_ZERO, 0, z);
+  philox_normal2(seed, id, ts, ST_ZERO, 1, z + 2);
+  double r2 = 0.0;
+  for (int d = 0; d < D; ++d) {
+    n[d] = z[d];
+    r2 += n[d] * n[d];
+  }
+  double norm = 1.0 / sqrt(r2);
+  for (int d = 0; d < D; ++d) n[d] *= norm;
+}
+
+__device__ inline void intra_cell(const StepParams& P, const Dev& S, int s, int c, int cslot, int ts,
+                                  double pos[2][3], double vel[2][3], const double* sig, double F[2][3],
+                                  bool* pos1_changed) {
+  const cil_species& sp = P.sp[s];
+  const int D = P.box.dim, nE = P.nE;
+  *pos1_changed = false;
+  if (nE == 2) {
+    double rij[3] = {0, 0, 0}, nij[3] = {0, 0, 0}, rnd[3] = {0, 0, 0};
+    double r2 = box_dist(P.box, rij, pos[0], pos[1]);
+    double r = sqrt(r2);
+    if (r2 == 0.0) {  // MD.cu:534-540
+      r = kBigTolMp * 0.5 * (sp.sigma[0] + sp.sigma[1]);
+      on_sphere_counter(P.seed, cell_key(s, c), (uint32_t)ts, D, rij);
+      for (int d = 0; d < D; ++d) rij[d] *= r;
+      r2 = sq(r);
+      for (int d = 0; d < D; ++d) pos[1][d] = box_update_dr1(P.box, d, pos[1][d], rij[d]);
+      *pos1_changed = true;
+    }
+    for (int d = 0; d < 3; ++d) nij[d] = rij[d];
+    double spring, scf;
+    if (sp.spring == CIL_SPRING_HARMONIC) {
+      double l = S.rest_len[cslot];
+      spring = -sp.k_harm * (r - l) / r;
+      scf = 1.0;
+      for (int d = 0; d < D; ++d) nij[d] = rij[d] / r;
+    } else {
+      double smax = sig[0] >= sig[1] ? sig[0] : sig[1];
+      double r2max = sp.r2max > smax ? sp.r2max : smax;
+      double ratio = sp.spring_ratio;
+      if (!(r2max - r2 > kTolMp)) atomicOr(S.err, (unsigned)ERR_BOND);
+      spring = r2 <= (r2max * ratio) ? sp.kappa_push * fabs(r2 - (r2max * ratio))
+                                     : -sp.kappa / (1.0 - sq((r2 - r2max * ratio) / (r2max * (1 - ratio))));
+      scf = 2.0;
+      if (!sp.cil) {
+        double scaling = sp.rss / r;
+        for (int d = 0; d < D; ++d) nij[d] *= scaling;
+      }
+    }
+    if (P.active_noise && D == 2) {
+      double Wr = S.awiener[(size_t)cslot * 2], Wt = S.awiener[(size_t)cslot * 2 + 1];
+      rnd[0] = 0.5 * (Wr * rij[0] / r - Wt * rij[1]);
+      rnd[1] = 0.5 * (Wr * rij[1] / r + Wt * rij[0]);
+    }
+    double a0 = sp.motility[0], a1 = sp.motility[1];
+    for (int d = 0; d < D; ++d) {
+      double tc0 = -(a0 * nij[d] + spring * rij[d] * scf + rnd[d] * rij[d]);
+      double tc1 = (a1 * nij[d] + spring * rij[d] * scf + rnd[d] * rij[d]);
+      if (sp.quad_drag) {
+        double v0 = vel[0][d], v1 = vel[1][d];
+        double tf0 = sp.zeta[0] * sq(v0) * (v0 / fabs(v0));
+        double tf1 = sp.zeta[1] * sq(v1) * (v1 / fabs(v1));
+        F[0][d] = (isinf(tf0) || isnan(tf0)) ? tc0 : (tc0 - tf0);
+        F[1][d] = (isinf(tf1) || isnan(tf1)) ? tc1 : (tc1 - tf1);
+      } else {
+        F[0][d] = tc0;
+        F[1][d] = tc1;
+      }
+    }
+  } else {
+    double vl = 0.0;
+    for (int d = 0; d < D; ++d) vl += vel[0][d] * vel[0][d];
+    vl = sqrt(vl);
+    if (vl == 0.0 && sp.motility[0] != 0.0) {  // MD.cu:609-614
+      on_sphere_counter(P.seed, cell_key(s, c) << 1, (uint32_t)ts, D, vel[0]);
+      for (int d = 0; d < D; ++d) vel[0][d] *= kBigTolMp;
+      vl = 0.0;
+      for (int d = 0; d < D; ++d) vl += vel[0][d] * vel[0][d];
+      vl = sqrt(vl);
+    }
+    for (int d = 0; d < D; ++d) F[0][d] = sp.motility[0] * (vl > 0.0 ? vel[0][d] / vl : 0.0);
+  }
+  for (int e = 0; e < nE; ++e)
+    for (int k = 0; k < P.nwalls; ++k) flat_wall(P.walls[k], sig[e], pos[e], F[e]);
+  if (P.noise == CIL_NOISE_CELL_WIENER)
+    for (int e = 0; e < nE; ++e)
+      for (int d = 0; d < D; ++d) F[e][d] += sp.zeta[e] * S.wiener[(size_t)cslot * 3 + d];
+}
+
+}  // namespace cil

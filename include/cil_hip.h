@@ -1,0 +1,253 @@
+/*
+ * cil_hip.h — C ABI of the MI355X (gfx950) overdamped cell-colony MD step.
+ *
+ * The library replaces the reference's Euler step (`void Euler(int& ts)`, CIL.cu:1013-1020),
+ * which the reference calls through the `MDIntegrator mdStep` function pointer
+ * (MD.cuh:450-452, invoked GTS times per frame at CIL.cu:1106).  A host program keeps
+ * its own per-species cell arrays (the reference's `CellColony`, Colony.hpp:81-113) and
+ * hands plain pointers to this ABI; device state stays resident between calls.
+ *
+ * Conventions
+ *  - Every function returns 0 (CIL_OK) or a negative cil_status; nothing throws.
+ *  - Host pointers are never cached across calls (the reference re-points its arrays on
+ *    x1.5 growth, Colony.cpp:193-273).
+ *  - Arrays use the reference's row-major layout: pos(cell, elem, d) at
+ *    (cell*nelem + elem)*dim + d (Array.h:557-600).
+ *  - One context = one colony (or one batch of colonies), one HIP stream, one host thread.
+ *  - Only plain C types cross the boundary.
+ */
+#ifndef CIL_HIP_H
+#define CIL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CIL_ABI_VERSION 1
+#define CIL_MAX_SPECIES 4
+#define CIL_MAX_ELEMS 2
+#define CIL_MAX_TYPES (CIL_MAX_SPECIES * CIL_MAX_ELEMS)
+#define CIL_MAX_WALLS 4
+
+typedef enum {
+  CIL_OK = 0,
+  CIL_E_INVALID = -1,     /* bad argument or parameter combination */
+  CIL_E_NO_DEVICE = -2,   /* no HIP device / runtime failure */
+  CIL_E_CAPACITY = -3,    /* colony grew beyond device capacity or host views */
+  CIL_E_OUTSIDE = -4,     /* particle outside the box and drain disabled (CIL.cu:955) */
+  CIL_E_NONFINITE = -5,   /* NaN/Inf found in positions or forces */
+  CIL_E_HIP = -6,         /* a HIP call failed; see cil_hip_last_error */
+  CIL_E_BOND = -7         /* FENE bond over-stretched (MD.cu:549 assert) */
+} cil_status;
+
+/* Pair-force law per element-type pair (reference: one global PairForceParams, MD.cuh:23-344;
+ * a per-type-pair matrix is an extension so that LJ and soft-core can be mixed). */
+typedef enum {
+  CIL_LAW_NONE = 0,
+  CIL_LAW_LJ = 1, /* LJParams  MD.cuh:47-129 */
+  CIL_LAW_SC = 2, /* SCParams  MD.cuh:134-232 */
+  CIL_LAW_HZ = 3, /* HZParams  MD.cuh:237-290, contact semantics of MD.cu:302-366 */
+  CIL_LAW_EA = 4  /* EAParams  MD.cuh:295-344, contact semantics of MD.cu:302-366 */
+} cil_law;
+
+typedef enum {
+  CIL_SPRING_FENE = 0,     /* reference FENE + push spring in r^2 (MD.cu:547-559) */
+  CIL_SPRING_HARMONIC = 1  /* extension: F = -k (r - l) r_hat, rest length l per cell */
+} cil_spring;
+
+typedef enum {
+  CIL_NOISE_NONE = 0,
+  CIL_NOISE_CELL_WIENER = 1, /* reference: one Wiener vector per cell, F_e += zeta_e W (CIL.cu:84-98) */
+  CIL_NOISE_FDT = 2          /* extension: per-disc dx += sqrt(2 kT mu dt) xi */
+} cil_noise;
+
+typedef enum {
+  CIL_RNG_COUNTER = 0,  /* Philox4x32-10 keyed by (seed, step, id, stream) */
+  CIL_RNG_EXTERNAL = 1  /* caller supplies the N(0,1) deviates (e.g. the reference's dSFMT stream) */
+} cil_rng;
+
+typedef enum {
+  CIL_DIV_NONE = 0,
+  CIL_DIV_THETA_CYCLE = 1,   /* calculDivisionsSingle, CIL.cu:469-619 (single-disc cells) */
+  CIL_DIV_LINEAR_GROWTH = 2  /* extension: two-disc rest length grows at rate 1/tau, divide at l_div */
+} cil_division;
+
+typedef enum {
+  CIL_PREC_FP64 = 0, /* forces in FP64 (parity mode) */
+  CIL_PREC_FP32 = 1  /* forces in FP32, positions accumulated in FP64 (fast mode) */
+} cil_precision;
+
+/* Normal-distribution parameters (reference `Gaussian`, Cell.cuh). */
+typedef struct {
+  double mu;
+  double sigma;
+} cil_gaussian;
+
+/* Per-species parameters; field names follow CellParams/ElemParams (Cell.cuh:32-266) after
+ * the unit scaling and derivations of readSpecs (Input.cpp:6-293). */
+typedef struct {
+  int nelem;                        /* 1 or 2; all species of a context share it */
+  double sigma[CIL_MAX_ELEMS];      /* SIGMA_CORE, default element diameters */
+  double zeta[CIL_MAX_ELEMS];       /* FRICTION */
+  double zeta_p[CIL_MAX_ELEMS];     /* P_FRICTION */
+  double zeta_p_beta[CIL_MAX_ELEMS];/* P_FRICTION_BETA */
+  double motility[CIL_MAX_ELEMS];   /* MOTILITY */
+  /* pair-law element parameters (meaning depends on the law, Input.cpp:48-82):
+   *   SC: eps_core = EPSILON_CORE, eps_well = EPSILON_RATIO*EPSILON_CORE, sig_well = SIGMA_RATIO*SIGMA_CORE
+   *   HZ/EA: eps_core = ELASTIC_MODULUS, eps_well = ADHESION_CONSTANT, sig_well = POISSON_RATIO
+   *   LJ: eps_well = EPSILON */
+  double eps_core[CIL_MAX_ELEMS];
+  double eps_well[CIL_MAX_ELEMS];
+  double sig_well[CIL_MAX_ELEMS];
+  double cross_adhesion;            /* CellParams::crossAdhesion */
+  /* intra-cell bond of two-disc cells */
+  int spring;                       /* cil_spring */
+  double kappa, kappa_push, spring_ratio, r2max, div_r2max; /* FENE block (Input.cpp:117-135) */
+  double rss;                       /* steady-state distance (Input.cpp:138-143) */
+  int cil;                          /* contact inhibition of locomotion flag */
+  int quad_drag;                    /* binaryCellForce's -zeta v^2 sign(v) term (MD.cu:588-595) */
+  double k_harm;                    /* harmonic extension: spring constant */
+  /* noise */
+  double d_noise;                   /* passive Wiener amplitude sqrt(2D)/sqrt(dt) (Input.cpp:108-114, CIL.cuh:210-215) */
+  double d_noise_radial, d_noise_angular; /* active noise amplitudes (two-disc, 2D) */
+  double kT;                        /* FDT extension */
+  /* theta cell cycle (calculDivisionsSingle, CIL.cu:469-619); tau_* already divided by dt */
+  double cycle_p0, cycle_tau_r, cycle_tau_v, cycle_tau_p, cycle_v_s, cycle_v_1, cycle_beta_vs;
+  double cycle_rmaxed_inv, cycle_rho_ratio, cycle_p_shift, cycle_p_shift_tau_inv, apop_pressure;
+  cil_gaussian division_tau;        /* log-normal division time (CIL.cu:591-592) */
+  cil_gaussian migration_tau;       /* normal switch time (CIL.cu:580-581) */
+  /* linear-growth extension (two-disc) */
+  double growth_rate;               /* d l / d t = growth_rate (= 1/tau) */
+  double l_div;                     /* divide when the rest length reaches l_div */
+  double l_birth;                   /* daughters' rest length */
+  double sigma_jitter;              /* daughter diameters sigma*(1 + jitter*N(0,1)) */
+} cil_species;
+
+typedef struct {
+  int kind;        /* 0 = FlatWall (Wall.hpp:62-101) */
+  int axis;        /* 0,1,2 */
+  double x0;
+  double rcut;     /* in units of sigma (default 2^(1/6), Input.cpp:381) */
+  double epsilon;
+  int positive;    /* wall pushes toward +axis */
+} cil_wall;
+
+typedef struct {
+  int abi_version;                  /* must be CIL_ABI_VERSION */
+  int dim;                          /* 2 or 3 */
+  int nspecies;
+  cil_species species[CIL_MAX_SPECIES];
+  /* element type t = species*nelem + elem; law and cut-off factor per type pair */
+  int law[CIL_MAX_TYPES][CIL_MAX_TYPES];
+  double rcut2[CIL_MAX_TYPES][CIL_MAX_TYPES]; /* PairForceParams::rcut2 (RCUT^2; default (2^(1/6))^2 LJ, 1 otherwise) */
+  int force_shifted;                /* OPT::FORCE_SHIFTED */
+  double dt;
+  int noise;                        /* cil_noise */
+  int active_noise;                 /* OPT::NOISE_ACTIVE */
+  int rng;                          /* cil_rng */
+  uint64_t seed;
+  int division;                     /* cil_division */
+  double osmotic_pi;                /* MDParams::pi */
+  int nwalls;
+  cil_wall walls[CIL_MAX_WALLS];
+  int drain;                        /* MDParams::drain (CIL.cu:945-966) */
+  int precision;                    /* cil_precision */
+  int pressure;                     /* accumulate pair pressure (needed when zeta_p != 0 or theta cycle) */
+} cil_params;
+
+typedef struct {
+  double length[3];
+  int periodic[3];
+} cil_box;
+
+/* View of one species' arrays in the host colony (CellColony, Colony.hpp:81-113).
+ * Optional arrays may be NULL.  `capacity` = number of cells the arrays can hold. */
+typedef struct {
+  int ncells;
+  int capacity;
+  double* pos;       /* ncells*nelem*dim */
+  double* frc;       /* ncells*nelem*dim   (download only) */
+  double* vel;       /* ncells*nelem*dim */
+  double* sigma;     /* ncells*nelem      sigmaCore */
+  double* sigma_t;   /* ncells            sigmaCoreT (theta cycle) */
+  double* pressure;  /* ncells */
+  int* state;        /* ncells CellState */
+  int* t0;           /* ncells */
+  int* tswitch;      /* ncells */
+  int* tpressure;    /* ncells */
+  double* theta;     /* ncells */
+  double* rho;       /* ncells */
+  double* ang_vel;   /* ncells */
+  double* rest_len;  /* ncells  (linear-growth / harmonic extension) */
+} cil_species_view;
+
+typedef struct cil_ctx cil_ctx;
+
+/* Create a context: device buffers sized for `capacity[s]` cells per species.
+ * Replaces the reference's initialize* chain for the step (CIL.cuh:332-393). */
+int cil_hip_create(const cil_params* params, const cil_box* box, const int* capacity, cil_ctx** out);
+int cil_hip_destroy(cil_ctx* ctx);
+
+/* Copy the host colony to the device (all species). */
+int cil_hip_upload(cil_ctx* ctx, const cil_species_view* views, int nspec);
+
+/* Run the Euler phase order (CIL.cu:1013-1020) `nsteps` times; *ts is incremented per step
+ * exactly like Euler(int& ts).  Requires forces to be initialised (cil_hip_forces or a
+ * previous step), as the reference does at CIL.cu:1096. */
+int cil_hip_step(cil_ctx* ctx, int nsteps, int* ts);
+
+/* Copy the device colony back.  If a species holds more cells than views[s].capacity,
+ * returns CIL_E_CAPACITY and writes the needed counts to required_cells[s]. */
+int cil_hip_download(cil_ctx* ctx, cil_species_view* views, int nspec, int* required_cells);
+
+/* calculForcesPressure at the current state (CIL.cu:999-1007; init call CIL.cu:1096).
+ * Draws noise when the noise mode is on.  If frc/pressure are non-NULL they receive the
+ * forces (UID order, FP64) and pressures. */
+int cil_hip_forces(cil_ctx* ctx, double* frc, double* pressure);
+
+/* Enumerate interacting element pairs (UID_i < UID_j, different cells, law predicate true)
+ * in ascending (i, j) order.  Writes up to `cap` pairs as ij[2k], ij[2k+1]; *n = total. */
+int cil_hip_pairs(cil_ctx* ctx, int32_t* ij, size_t cap, size_t* n);
+
+/* checkPositions (CIL.cu:945-966): out-of-box cells raise CIL_E_OUTSIDE unless drain,
+ * in which case they are removed by swap-with-last with the reference's skip quirk. */
+int cil_hip_check_positions(cil_ctx* ctx, int* removed);
+
+/* Supply external N(0,1) deviates for CIL_RNG_EXTERNAL: `nsteps` blocks of `per_step`
+ * doubles in the reference's draw order (CIL.cu:793-818: all species' passive vectors,
+ * species-major, cell-major, dim-minor; then active radial and angular blocks).  Block k
+ * feeds the noise draw at step index ts + k, where ts is the context's current step index
+ * (the draw of cil_hip_forces is at ts, the draw of the j-th following step at ts + j). */
+int cil_hip_set_external_noise(cil_ctx* ctx, const double* normals, size_t per_step, int nsteps);
+
+/* Current number of cells per species (device truth). */
+int cil_hip_num_cells(cil_ctx* ctx, int* ncells);
+
+/* Time spent per kernel family over the last cil_hip_step call, in microseconds
+ * (measured with HIP events on the context stream); `names` is a ';'-separated list. */
+int cil_hip_timings(cil_ctx* ctx, double* us, int cap, int* n, char* names, size_t names_cap);
+
+/* Bin geometry in use: number of bins per dimension and bin width. */
+int cil_hip_bin_info(cil_ctx* ctx, int* nbins, double* width);
+
+/* Autotuner knobs (project addition): bins per cut-off (1..4) and threads per block of the
+ * pair kernel (64, 128 or 256; 0 keeps the current value). */
+int cil_hip_set_tuning(cil_ctx* ctx, int bins_per_cutoff, int block);
+
+/* Per-kernel-family HIP-event timing of eager steps (project addition; off by default). */
+int cil_hip_set_timing(cil_ctx* ctx, int on);
+
+/* sizeof the ABI structs (0: cil_params, 1: cil_species, 2: cil_species_view, 3: cil_box),
+ * so an FFI binding can verify its layout (project addition). */
+size_t cil_hip_sizeof(int which);
+
+const char* cil_hip_last_error(cil_ctx* ctx);
+const char* cil_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CIL_HIP_H */

@@ -1,0 +1,273 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on identical inputs.
+
+Bars (written here, per the method's contract):
+  - interacting pair sets and division / removal events: bit-exact;
+  - per-particle forces: |F_gpu - F_ref| <= tol * (S_p + |F_ref|), tol = 1e-12 (FP64 mode)
+    or 1e-5 (FP32 mode), S_p = sum over the particle's pair terms |f0 r_ij|;
+  - trajectories (noise-free, counter-noise, dSFMT replay): max displacement difference
+    <= 1e-9 sigma over 10^3 steps.
+The oracle uses its all-pairs traversal (canonical i<j orientation, CIL.cu:232-263).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from cil_cuda_amd import abi, configs
+from cil_cuda_amd.colony import HipMD
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _minimg(d, box, dim):
+    d = d.copy()
+    for k in range(dim):
+        if box.periodic[k]:
+            L = box.length[k]
+            d[..., k] -= L * np.round(d[..., k] / L)
+    return d
+
+
+def _random_colony(dim, nelem, n, L, periodic, law, seed, nspec=2):
+    rng = np.random.default_rng(seed)
+    p = configs.base_params(dim, nspec, nelem)
+    for s in range(nspec):
+        for e in range(nelem):
+            p.species[s].sig_well[e] = 0.5 + 0.2 * s
+            p.species[s].eps_core[e] = 10.0
+            p.species[s].eps_well[e] = 1.0
+    configs.set_law(p, law, 2.0 if law == abi.LAW_LJ else 1.0)
+    box = configs.make_box([L] * dim, [periodic] * dim)
+    per = n // nspec
+    from cil_cuda_amd.colony import Colony
+
+    col = Colony(dim, nelem, [per] * nspec, ncells=[per] * nspec)
+    for sp in col.species:
+        c = rng.uniform(0, L, (per, dim))
+        sp["pos"][:, 0] = c
+        if nelem == 2:
+            u = rng.normal(size=(per, dim))
+            u /= np.linalg.norm(u, axis=1, keepdims=True)
+            sp["pos"][:, 1] = np.mod(c + 0.6 * u, L) if periodic else c + 0.6 * u
+        sp["sigma"][:] = rng.uniform(0.8, 1.25, (per, nelem))
+        sp["rest_len"][:] = 0.6
+        sp["state"][:] = abi.CS_CRAWLING
+    return p, box, col
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("periodic", [1, 0])
+@pytest.mark.parametrize("nelem", [1, 2])
+@pytest.mark.parametrize("law", [abi.LAW_LJ, abi.LAW_SC, abi.LAW_HZ])
+def test_pairs_bit_exact(gpu, dim, periodic, nelem, law):
+    n = 400 if dim == 2 else 600
+    L = 18.0 if dim == 2 else 9.0
+    say(f"pairs dim={dim} pbc={periodic} nelem={nelem} law={law}")
+    p, box, col = _random_colony(dim, nelem, n, L, periodic, law, seed=100 + dim * 10 + nelem + law)
+    ref = O.Oracle(p, box, traversal=1).pairs(col, mode=1)
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    got = md.pairs()
+    md.close()
+    assert len(ref) > 0
+    assert np.array_equal(got, ref)
+
+
+def _force_parity(p, box, col, tol, ts=0):
+    orc = O.Oracle(p, box, traversal=1)
+    ref = col.copy()
+    orc.forces(ref, ts)
+    scale = orc.force_scale(ref)
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    frc, pr = md.forces()
+    md.close()
+    fref = ref.flat("frc").reshape(-1, col.dim)
+    err = np.linalg.norm(frc - fref, axis=1)
+    bound = tol * (scale + np.linalg.norm(fref, axis=1)) + 1e-300
+    worst = np.max(err / bound)
+    return worst, frc, fref, pr, ref.flat("pressure")
+
+
+@pytest.mark.parametrize("precision,tol", [(abi.PREC_FP64, 1e-12), (abi.PREC_FP32, 1e-5)])
+def test_forces_plain_2d(gpu, precision, tol):
+    say("forces plain_2d precision", precision)
+    p, box, col = configs.plain_2d_sc(n_cells=300, L=24.0, seed=2, precision=precision)
+    worst, *_ = _force_parity(p, box, col, tol)
+    assert worst <= 1.0, worst
+
+
+@pytest.mark.parametrize("law", [abi.LAW_LJ, abi.LAW_SC, abi.LAW_HZ, abi.LAW_EA])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_forces_laws_fp64(gpu, law, dim):
+    say("forces law", law, "dim", dim)
+    p, box, col = _random_colony(dim, 1, 300 if dim == 2 else 500, 16.0 if dim == 2 else 8.0, 1, law, seed=7 + law)
+    p.pressure = 1
+    p.species[1].cross_adhesion = 0.5  # asymmetric adhesion scaling between species
+    worst, frc, fref, pr, prr = _force_parity(p, box, col, 1e-12)
+    assert worst <= 1.0, worst
+    if law in (abi.LAW_HZ, abi.LAW_EA):
+        assert np.allclose(pr, prr, rtol=1e-11, atol=1e-11 * np.max(np.abs(prr)))
+
+
+@pytest.mark.parametrize("law", [abi.LAW_LJ, abi.LAW_SC, abi.LAW_HZ])
+def test_forces_laws_fp32(gpu, law):
+    say("forces fp32 law", law)
+    p, box, col = _random_colony(3, 1, 800, 9.0, 1, law, seed=17 + law)
+    p.precision = abi.PREC_FP32
+    worst, *_ = _force_parity(p, box, col, 1e-5)
+    assert worst <= 1.0, worst
+
+
+def _trajectory(p, box, col, nsteps, ext=None):
+    orc = O.Oracle(p, box, traversal=1)
+    ref = col.copy()
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    if ext is not None:
+        orc.set_external_noise(ext)
+        md.set_external_noise(ext)
+    orc.forces(ref, 0)
+    md.forces(want=False)
+    ts_r = orc.step(ref, nsteps, 0)
+    ts_g = md.step(nsteps, 0)
+    assert ts_r == ts_g == nsteps
+    got = col.copy()
+    md.download(got)
+    md.close()
+    return ref, got
+
+
+def _max_disp_err(a, b, box):
+    assert a.ncells() == b.ncells()
+    pa = a.flat("pos").reshape(-1, a.dim)
+    pb = b.flat("pos").reshape(-1, b.dim)
+    return float(np.max(np.linalg.norm(_minimg(pa - pb, box, a.dim), axis=1)))
+
+
+@pytest.mark.parametrize("noise", [abi.NOISE_NONE, abi.NOISE_FDT])
+def test_trajectory_plain_2d_1000_steps(gpu, noise):
+    say("trajectory noise", noise)
+    p, box, col = configs.plain_2d_sc(n_cells=200, L=20.0, seed=4, noise=noise)
+    ref, got = _trajectory(p, box, col, 1000)
+    err = _max_disp_err(ref, got, box)
+    say("max displacement error", err)
+    assert err <= 1e-9
+
+
+def test_trajectory_dsfmt_replay(gpu):
+    # reference noise: one Wiener vector per cell drawn from the reference's dSFMT stream
+    say("trajectory dsfmt replay")
+    p, box, col = configs.plain_2d_sc(n_cells=150, L=18.0, seed=5, noise=abi.NOISE_CELL_WIENER)
+    p.species[0].d_noise = math.sqrt(2 * 0.01) / math.sqrt(p.dt)
+    p.rng = abi.RNG_EXTERNAL
+    nsteps = 1000
+    per = 150 * 2
+    z = O.dsfmt_fill(1, 1, per * (nsteps + 1)).reshape(nsteps + 1, per)
+    ref, got = _trajectory(p, box, col, nsteps, ext=z)
+    err = _max_disp_err(ref, got, box)
+    say("max displacement error", err)
+    assert err <= 1e-9
+
+
+def test_trajectory_reference_fene_cells(gpu):
+    # reference two-disc cells: FENE + push spring in r^2, rss motility, quadratic drag
+    say("trajectory fene")
+    p, box, col = configs.plain_2d_sc(n_cells=150, L=18.0, seed=6, noise=abi.NOISE_NONE)
+    sp = p.species[0]
+    sp.spring = abi.SPRING_FENE
+    sp.kappa = 1.0
+    sp.kappa_push = 5.0
+    sp.spring_ratio = 0.3
+    sp.r2max = 1.0
+    sp.rss = 0.5
+    sp.quad_drag = 1
+    sp.motility[0] = 0.05
+    ref, got = _trajectory(p, box, col, 500)
+    err = _max_disp_err(ref, got, box)
+    say("max displacement error", err)
+    assert err <= 1e-9
+
+
+def test_theta_cycle_divisions_bit_exact(gpu):
+    say("theta divisions")
+    p, box, col = configs.theta_2d_hz(n_cells=12, L=30.0, seed=3)
+    ref, got = _trajectory(p, box, col, 200)
+    assert ref.ncells() == got.ncells()
+    assert ref.ncells()[0] > 12, "no division happened"
+    a, b = ref.species[0], got.species[0]
+    n = a["ncells"]
+    for f in ("state", "t0", "tswitch", "tpressure"):
+        assert np.array_equal(a[f][:n], b[f][:n]), f
+    for f in ("theta", "rho", "ang_vel", "sigma_t"):
+        assert np.allclose(a[f][:n], b[f][:n], rtol=1e-11, atol=1e-12), f
+    assert _max_disp_err(ref, got, box) <= 1e-9
+
+
+def test_linear_growth_divisions_and_drain(gpu):
+    say("linear growth + drain")
+    p, box, col = configs.channel_2d_hz(width=12.0, length=30.0, capacity=512, dt=1e-3, tau=0.05)
+    ref, got = _trajectory(p, box, col, 300)
+    assert ref.ncells() == got.ncells()
+    assert ref.ncells()[0] > 8
+    n = ref.ncells()[0]
+    assert np.array_equal(ref.species[0]["t0"][:n], got.species[0]["t0"][:n])
+    assert np.allclose(ref.species[0]["rest_len"][:n], got.species[0]["rest_len"][:n], atol=1e-12)
+    assert _max_disp_err(ref, got, box) <= 1e-9
+
+
+def test_drain_removal_order(gpu):
+    say("drain removal")
+    p, box, col = configs.channel_2d_hz(width=12.0, length=30.0, capacity=64, dt=1e-3)
+    p.division = abi.DIV_NONE
+    rng = np.random.default_rng(9)
+    s = col.species[0]
+    n = 40
+    s["ncells"] = n
+    c = np.stack([rng.uniform(1, 11, n), rng.uniform(-3, 33, n)], axis=1)
+    s["pos"][:n, 0] = c
+    s["pos"][:n, 1] = c + [0.0, 0.5]
+    s["sigma"][:n] = 1.0
+    s["rest_len"][:n] = 0.5
+    s["tswitch"][:n] = np.arange(n)  # tag cells to follow the moves
+    orc = O.Oracle(p, box, traversal=1)
+    ref = col.copy()
+    orc.forces(ref, 0)
+    removed_ref = orc.check_positions(ref)
+    md = HipMD(p, box, [64])
+    md.upload(col)
+    md.forces(want=False)
+    removed = md.check_positions()
+    got = col.copy()
+    md.download(got)
+    md.close()
+    assert removed == removed_ref > 0
+    m = ref.species[0]["ncells"]
+    assert got.species[0]["ncells"] == m
+    assert np.array_equal(got.species[0]["tswitch"][:m], ref.species[0]["tswitch"][:m])
+
+
+def test_bulk3d_fp32_against_fp64(gpu):
+    say("bulk 3d fp32 vs fp64")
+    p, box, col = configs.bulk_3d_mixed(n=8192, seed=2, precision=abi.PREC_FP64)
+    # push-off relaxation on the GPU, then compare force modes on the relaxed state
+    q = configs.relaxation_params(p)
+    md = HipMD(q, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    md.forces(want=False)
+    md.step(100, 0)
+    md.download(col)
+    md.close()
+    worst, *_ = _force_parity(p, box, col, 1e-12)
+    assert worst <= 1.0
+    p.precision = abi.PREC_FP32
+    worst32, *_ = _force_parity(p, box, col, 1e-5)
+    say("fp32 worst", worst32)
+    assert worst32 <= 1.0
