@@ -1,0 +1,152 @@
+#!/usr/bin/env python3
+"""Benchmark: plain FP32 colony step at 1,048,576 particles (3D periodic mixed LJ / soft-core
+bulk, lognormal diameters, packing fraction 0.6), one MI355X.
+
+Prints ONE JSON line (rank 0) with particle-steps/s, the roofline of the dominant kernel
+(measured with HIP events inside the library on the step stream over the timed region) and
+the CPU baseline (the oracle's reference-structured step on a bounded sample of the same
+relaxed state, timed on this host).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from cil_cuda_amd import abi, configs  # noqa: E402
+from cil_cuda_amd.colony import HipMD  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP32_VALU_PEAK_TFLOPS = 157.3
+N_PARTICLES = 1 << 20
+
+# Algorithmic bytes per particle per launch (external inputs once + outputs once) of the
+# pair kernel in 3D FP32: read sorted x,y,z,sigma (16) + tag (4); write force (12) + pressure (4).
+PAIR_BYTES_PER_PARTICLE = 36
+# Whole FP32 3D step (update+keys 72, scan 4, scatter+binsort 60, pair 36)
+STEP_BYTES_PER_PARTICLE = 200
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def relaxed_state(n, seed=1):
+    p, box, col = configs.bulk_3d_mixed(n=n, seed=seed, precision=abi.PREC_FP32)
+    q = configs.relaxation_params(p)
+    md = HipMD(q, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    md.forces(want=False)
+    md.step(100, 0)
+    md.download(col)
+    md.close()
+    return p, box, col
+
+
+def cpu_baseline(seed=1, n_sample=1 << 17):
+    """Time the oracle's reference-structured step (COM link list, per-cell-pair element
+    loops, -O3 -ffast-math like the reference's -Ofast) on this host, on a bounded sample:
+    a 131,072-particle bulk drawn and relaxed exactly like the benchmark workload (same
+    density, diameter distribution, law mix).  All host cores (race-free gather) and 1 core."""
+    from oracle import oracle as O
+
+    p, box, col = relaxed_state(n_sample, seed)
+    q = abi.Params.from_buffer_copy(p)
+    q.precision = abi.PREC_FP64
+    threads = max(1, min(16, os.cpu_count() or 1))
+    res = {}
+    for nth, nsteps in ((threads, 3), (1, 1)):
+        orc = O.Oracle(q, box, fast=True, traversal=0, threads=nth)
+        c = col.copy()
+        orc.forces(c, 0)
+        t0 = time.perf_counter()
+        orc.step(c, nsteps, 0)
+        dt = time.perf_counter() - t0
+        orc.close()
+        res[nth] = (n_sample * nsteps / dt, nsteps, dt)
+        log(f"cpu baseline: {nth} thread(s) {res[nth][0]:.3e} particle-steps/s ({nsteps} steps, {dt:.1f} s)")
+    v, nsteps, dt = res[threads]
+    return {"value": v, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{nsteps} Euler steps of a {n_sample}-particle bulk drawn and relaxed like the 1M workload "
+                      f"(oracle -O3 -ffast-math, COM link list, race-free gather over {threads} threads) in {dt:.1f} s",
+            "value_1core": res[1][0]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--n", type=int, default=N_PARTICLES)
+    ap.add_argument("--bins-per-cutoff", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the multi-GPU "
+              "sharding is a design (see DESIGN.md), not a run", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    p, box, col = relaxed_state(args.n)
+    log(f"relaxed {args.n} particles in {time.perf_counter() - t0:.1f} s")
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    if args.bins_per_cutoff:
+        md.set_tuning(args.bins_per_cutoff, 0)
+    md.upload(col)
+    md.forces(want=False)
+    ts = md.step(args.warmup, 0)
+    md.set_timing(True)
+    md.num_cells()  # synchronise
+    log(f"warm-up done ({args.warmup} steps); timing {args.steps} steps")
+    t0 = time.perf_counter()
+    ts = md.step(args.steps, ts)
+    md.num_cells()
+    elapsed = time.perf_counter() - t0
+    log(f"timed region: {elapsed * 1e3 / args.steps:.3f} ms/step")
+    fam = md.timings()
+    nb, w = md.bin_info()
+    md.set_timing(False)
+    n = args.n
+    value = n * args.steps / elapsed
+    pair_us = fam.get("pair", 0.0)
+    achieved = PAIR_BYTES_PER_PARTICLE * n / (pair_us * 1e-6) / 1e9 if pair_us > 0 else 0.0
+    out = {
+        "metric": "particle-steps/s (plain FP32 step, 3D bulk, 1M particles)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 forces / fp64 positions",
+        "data": "synthetic (seeded lognormal bulk, 100 push-off relaxation steps)",
+        "config": {"workload": "3D periodic bulk, 1048576 single discs, phi=0.6, lognormal sigma CV 0.1, "
+                               "LJ 2.5 sigma (species 0 pairs) + soft core, dt=1e-4, kT=0.05",
+                   "particles": n, "bins": nb, "bin_width": w},
+        "kernels_us": fam,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_pair<float>",
+                     "bytes_per_particle": PAIR_BYTES_PER_PARTICLE},
+        "step_hbm_gbs": STEP_BYTES_PER_PARTICLE * value / 1e9,
+    }
+    md.close()
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline()
+        except Exception as e:  # the baseline must not hide the GPU number
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -82,6 +82,7 @@ class Params(C.Structure):
         ("osmotic_pi", C.c_double),
         ("nwalls", C.c_int), ("walls", Wall * MAX_WALLS),
         ("drain", C.c_int), ("precision", C.c_int), ("pressure", C.c_int),
+        ("max_disp", C.c_double),
     ]
 
 

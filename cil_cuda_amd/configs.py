@@ -185,20 +185,15 @@ def bulk_3d_mixed(n=1 << 20, phi=0.6, seed=1, kT=0.05, dt=1e-4, precision=abi.PR
     return p, box, col
 
 
-def relaxation_params(p: abi.Params, dt=1e-2) -> abi.Params:
-    """Push-off protocol for random initial positions: soft core for every pair (bounded
-    force), larger dt, no noise.  Used for the 100 relaxation steps of the 3D bulk."""
+def relaxation_params(p: abi.Params, dt=1e-3, max_disp=0.05) -> abi.Params:
+    """Relaxation of random initial positions: the production laws, a 10x larger time step,
+    no noise, and every displacement capped at `max_disp` sigma per step (the restated laws all
+    vanish or diverge at full overlap, so an uncapped push-off cannot separate coincident
+    discs).  Used for the 100 relaxation steps of the 3D bulk."""
     q = abi.Params.from_buffer_copy(p)
-    nt = q.nspecies * q.species[0].nelem
-    for i in range(nt):
-        for j in range(nt):
-            q.law[i][j] = abi.LAW_SC
-            q.rcut2[i][j] = 1.0
-    for s in range(q.nspecies):
-        q.species[s].eps_well[0] = 0.0
-        q.species[s].eps_well[1] = 0.0
     q.dt = dt
     q.noise = abi.NOISE_NONE
+    q.max_disp = max_disp
     return q
 
 

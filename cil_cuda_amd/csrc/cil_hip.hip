@@ -685,12 +685,22 @@ __global__ void __launch_bounds__(256) k_update(const StepParams* __restrict__ P
     }
     double amp = fdt ? sqrt(2.0 * sp.kT * temp) : 0.0;
     double xn[3];
+    if (fdt || P.max_disp > 0.0) {
+      double dx[3] = {0, 0, 0}, n2 = 0.0;
+      for (int d = 0; d < D; ++d) {
+        dx[d] = fdt ? temp * F[e][d] + amp * z[d] : temp * F[e][d];
+        n2 += dx[d] * dx[d];
+      }
+      if (P.max_disp > 0.0 && n2 > P.max_disp * P.max_disp) {  // relaxation extension
+        double f = P.max_disp / sqrt(n2);
+        for (int d = 0; d < D; ++d) dx[d] *= f;
+      }
+      for (int d = 0; d < D; ++d) xn[d] = box_update_dr1(P.box, d, pos[e][d], dx[d]);
+    } else {
+      for (int d = 0; d < D; ++d) xn[d] = box_update1(P.box, d, pos[e][d], F[e][d], temp);
+    }
     for (int d = 0; d < D; ++d) {
-      double x;
-      if (fdt) x = box_update_dr1(P.box, d, pos[e][d], temp * F[e][d] + amp * z[d]);
-      else x = box_update1(P.box, d, pos[e][d], F[e][d], temp);
-      xn[d] = x;
-      S.pos[(size_t)d * S.P + p] = x;
+      S.pos[(size_t)d * S.P + p] = xn[d];
       S.vel[(size_t)d * S.P + p] = F[e][d] * mu;
     }
     if (emit_keys) {

@@ -153,6 +153,7 @@ struct StepParams {
   int noise, active_noise, rng, division, nwalls, drain, precision, pressure;
   uint64_t seed;
   double osmotic_pi;
+  double max_disp;
   cil_wall walls[CIL_MAX_WALLS];
   cil_species sp[CIL_MAX_SPECIES];
   // bins

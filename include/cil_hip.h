@@ -156,6 +156,8 @@ typedef struct {
   int drain;                        /* MDParams::drain (CIL.cu:945-966) */
   int precision;                    /* cil_precision */
   int pressure;                     /* accumulate pair pressure (needed when zeta_p != 0 or theta cycle) */
+  double max_disp;                  /* extension for relaxing random initial states: per-step displacement
+                                       cap (0 = off, the reference behaviour) */
 } cil_params;
 
 typedef struct {

@@ -905,6 +905,7 @@ static void intra_forces(or_ws* w, cil_species_view* v, int nspec, int ts) { /* 
   int s, c, e, d, k, D = w->box.dim, nE = w->nE;
   for (s = 0; s < nspec; s++) {
     const cil_species* P = &w->p.species[s];
+#pragma omp parallel for num_threads(w->nthreads) if (w->nthreads > 1 && w->rng_mode != OR_RNG_DSFMT)
     for (c = 0; c < v[s].ncells; c++) {
       if (nE == 2) binary_cell_force(w, v, s, c, ts);
       else single_cell_force(w, v, s, c, ts);
@@ -960,6 +961,65 @@ static int draw_noise(or_ws* w, cil_species_view* v, int nspec, int ts) {
   return 0;
 }
 
+/* Race-free multi-threaded variant of the link-list traversal (CIL.cu:177-231 has OpenMP
+ * pragmas but writes both cells of a pair; this gathers instead): every cell visits the full
+ * 3^D box shell around its own box and accumulates only its own force and pressure.  Pair
+ * terms are evaluated in canonical orientation (lower UID first).  Counter RNG only. */
+static void inter_forces_gather(or_ws* w, cil_species_view* v, int nspec, const uidmap* um, int ts) {
+  int D = w->box.dim, nE = w->nE, ncellsTot = um->cumul[nspec], nboxes = 0;
+  int disabled = build_linklist(w, v, nspec, um, &nboxes);
+  int ii;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(w->nthreads)
+  for (ii = 0; ii < ncellsTot; ii++) {
+    int si, ci, icell[3] = {0, 0, 0}, ox, oy, oz, d, ib = 0;
+    double dumf[3 * CIL_MAX_ELEMS], dump = 0.0;
+    uid2cell(um, ii, &si, &ci);
+    if (!disabled) {
+      const double* cm = &w->com[(size_t)ii * D];
+      for (d = 0; d < D; d++) {
+        int b = (int)((cm[d] * (1.0 / w->box.L[d])) * w->ns[d]);
+        icell[d] = b < w->ns[d] - 1 ? b : w->ns[d] - 1;
+      }
+      ib = D == 2 ? icell[0] * w->ns[1] + icell[1] : (icell[0] * w->ns[1] + icell[1]) * w->ns[2] + icell[2];
+    }
+    for (ox = -1; ox <= 1; ox++)
+      for (oy = -1; oy <= 1; oy++)
+        for (oz = (D == 3 ? -1 : 0); oz <= (D == 3 ? 1 : 0); oz++) {
+          int jj, jstart;
+          if (!disabled) {
+            int dxyz[3] = {ox, oy, oz}, jc;
+            /* skip offsets that wrap onto an already visited box (ns == 3 is fine) */
+            jc = neighbor_id(w, icell, dxyz);
+            if (jc == -1) continue;
+            jstart = w->head[jc];
+          } else {
+            if (ox != 0 || oy != 0 || oz != 0) continue;
+            jstart = 0;
+          }
+          for (jj = jstart; jj >= 0 && jj < ncellsTot; jj = disabled ? jj + 1 : w->list[jj]) {
+            int sj, cj, ei, ej;
+            if (jj == ii) continue;
+            uid2cell(um, jj, &sj, &cj);
+            for (ei = 0; ei < nE; ei++)
+              for (ej = 0; ej < nE; ej++) {
+                size_t pi = (size_t)ci * nE + ei, pj = (size_t)cj * nE + ej;
+                uint32_t ki = (uint32_t)((cell_key(si, ci) << 1) | ei), kj = (uint32_t)((cell_key(sj, cj) << 1) | ej);
+                for (d = 0; d < 3 * CIL_MAX_ELEMS; d++) dumf[d] = 0.0;
+                if (ii < jj)
+                  elem_pair(w, si * nE + ei, sj * nE + ej, si, sj, &v[si].pos[pi * D], &v[sj].pos[pj * D],
+                            v[si].sigma[pi], v[sj].sigma[pj], &v[si].frc[pi * D], dumf, &v[si].pressure[ci], &dump, NULL,
+                            NULL, ki, kj, ts);
+                else
+                  elem_pair(w, sj * nE + ej, si * nE + ei, sj, si, &v[sj].pos[pj * D], &v[si].pos[pi * D],
+                            v[sj].sigma[pj], v[si].sigma[pi], dumf, &v[si].frc[pi * D], &dump, &v[si].pressure[ci], NULL,
+                            NULL, kj, ki, ts);
+              }
+          }
+          (void)ib;
+        }
+  }
+}
+
 /* calculInterForces, CIL.cu:108-264 */
 static void inter_forces(or_ws* w, cil_species_view* v, int nspec, int ts) {
   uidmap um;
@@ -967,6 +1027,10 @@ static void inter_forces(or_ws* w, cil_species_view* v, int nspec, int ts) {
   make_uidmap(&um, v, nspec);
   for (s = 0; s < nspec; s++)
     for (c = 0; c < v[s].ncells; c++) v[s].pressure[c] = 0;
+  if (w->nthreads > 1 && w->traversal == 0 && w->rng_mode == OR_RNG_COUNTER) {
+    inter_forces_gather(w, v, nspec, &um, ts);
+    return;
+  }
   traverse(w, v, nspec, &um, force_cb, &ts);
 }
 
@@ -984,12 +1048,25 @@ int or_forces(or_ws* w, cil_species_view* v, int nspec, int ts) {
 /* ===================================================================================== */
 /* Update (CIL.cu:902-939) and centres of mass (Colony.cpp:409-432)                       */
 /* ===================================================================================== */
+/* Relaxation extension: scale a displacement down to length `cap` (cap <= 0: no-op). */
+static void cap_disp(double cap, double* dx, int D) {
+  double n2 = 0.0;
+  int d;
+  if (!(cap > 0.0)) return;
+  for (d = 0; d < D; d++) n2 += dx[d] * dx[d];
+  if (n2 > cap * cap) {
+    double f = cap / sqrt(n2);
+    for (d = 0; d < D; d++) dx[d] *= f;
+  }
+}
+
 static void update_positions(or_ws* w, cil_species_view* v, int nspec, int ts) {
   int s, c, e, d, D = w->box.dim, nE = w->nE;
   double dt = w->p.dt;
   for (s = 0; s < nspec; s++) {
     const cil_species* P = &w->p.species[s];
     double p0 = P->cycle_p0;
+#pragma omp parallel for private(e, d) num_threads(w->nthreads) if (w->nthreads > 1)
     for (c = 0; c < v[s].ncells; c++) {
       double pi = v[s].pressure[c];
       for (e = 0; e < nE; e++) {
@@ -1007,6 +1084,12 @@ static void update_positions(or_ws* w, cil_species_view* v, int nspec, int ts) {
           or_philox_normal2(w->p.seed, (uint32_t)((cell_key(s, c) << 1) | e), (uint32_t)ts, ST_FDT, 0, z);
           or_philox_normal2(w->p.seed, (uint32_t)((cell_key(s, c) << 1) | e), (uint32_t)ts, ST_FDT, 1, z + 2);
           for (d = 0; d < D; d++) dx[d] = temp * F[d] + amp * z[d];
+          cap_disp(w->p.max_disp, dx, D);
+          box_update_pos_dr(&w->box, R, dx);
+        } else if (w->p.max_disp > 0.0) {
+          double dx[3];
+          for (d = 0; d < D; d++) dx[d] = temp * F[d];
+          cap_disp(w->p.max_disp, dx, D);
           box_update_pos_dr(&w->box, R, dx);
         } else {
           box_update_pos(&w->box, R, F, temp);

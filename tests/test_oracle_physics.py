@@ -279,3 +279,18 @@ def test_com_of_two_disc_cell_across_boundary():
     orc = O.Oracle(p, box)
     com = orc.coms(col)
     assert com[0] == pytest.approx([0.1, 5.0], abs=1e-12)
+
+
+@pytest.mark.parametrize("dim,nelem", [(2, 2), (3, 1)])
+def test_threaded_gather_matches_serial_traversal(dim, nelem):
+    p, box, col = _random_colony(dim, nelem, 600 if dim == 2 else 1500, 30.0 if dim == 2 else 14.0, 1, abi.LAW_SC,
+                                 seed=21, nspec=2)
+    p.pressure = 1
+    a, b = col.copy(), col.copy()
+    O.Oracle(p, box, traversal=0, threads=1).forces(a, 0)
+    orc = O.Oracle(p, box, traversal=0, threads=4)
+    orc.forces(b, 0)
+    assert orc.linklist_info()[0][0] >= 3
+    fa, fb = a.flat("frc"), b.flat("frc")
+    assert np.max(np.abs(fa - fb)) <= 1e-12 * np.max(np.abs(fa))
+    assert np.allclose(a.flat("pressure"), b.flat("pressure"), rtol=1e-12, atol=1e-12)
