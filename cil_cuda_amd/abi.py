@@ -139,6 +139,8 @@ def lib():
     L.cil_hip_bin_info.argtypes = [vp, PI, PD]
     L.cil_hip_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.cil_hip_set_timing.argtypes = [vp, C.c_int]
+    L.cil_hip_set_kernel.argtypes = [vp, C.c_int]
+    L.cil_hip_autotune.argtypes = [vp, C.c_int, PI, PI, PI, PD]
     L.cil_hip_last_error.argtypes = [vp]
     L.cil_hip_last_error.restype = C.c_char_p
     L.cil_hip_version.restype = C.c_char_p
@@ -161,6 +163,6 @@ def check_layout(L=None):
 EXPORTS = (
     "cil_hip_create", "cil_hip_destroy", "cil_hip_upload", "cil_hip_step", "cil_hip_download",
     "cil_hip_forces", "cil_hip_pairs", "cil_hip_check_positions", "cil_hip_set_external_noise",
-    "cil_hip_num_cells", "cil_hip_timings", "cil_hip_bin_info", "cil_hip_set_tuning", "cil_hip_set_timing",
+    "cil_hip_num_cells", "cil_hip_timings", "cil_hip_bin_info", "cil_hip_set_tuning", "cil_hip_set_timing", "cil_hip_set_kernel", "cil_hip_autotune",
     "cil_hip_sizeof", "cil_hip_last_error", "cil_hip_version",
 )

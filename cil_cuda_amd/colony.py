@@ -187,6 +187,17 @@ class HipMD:
     def set_tuning(self, bins_per_cutoff=0, block=0):
         _check(self.L, self.ctx, self.L.cil_hip_set_tuning(self.ctx, int(bins_per_cutoff), int(block)), "set_tuning")
 
+    def autotune(self, reps=5):
+        """Pick bins per cut-off, pair block size and kernel variant by timing force passes."""
+        b, k, v = C.c_int(), C.c_int(), C.c_int()
+        us = C.c_double()
+        _check(self.L, self.ctx, self.L.cil_hip_autotune(self.ctx, int(reps), C.byref(b), C.byref(k), C.byref(v),
+                                                         C.byref(us)), "autotune")
+        return {"bins_per_cutoff": b.value, "block": k.value, "variant": v.value, "force_pass_us": us.value}
+
+    def set_kernel(self, variant):
+        _check(self.L, self.ctx, self.L.cil_hip_set_kernel(self.ctx, int(variant)), "set_kernel")
+
     def set_timing(self, on=True):
         _check(self.L, self.ctx, self.L.cil_hip_set_timing(self.ctx, 1 if on else 0), "set_timing")
 

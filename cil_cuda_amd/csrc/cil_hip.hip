@@ -10,10 +10,11 @@
 //   [k_noise]    Wiener deviates per cell (CIL.cu:793-818), reference noise mode only
 //   k_scan_*     exclusive prefix sum of bin counts
 //   k_scatter    particle -> slot in bin order
-//   k_binsort    one wavefront per bin: bitonic sort of the bin's particle ids in LDS (so
-//                the order is deterministic), gather of the bin-local coordinates
-//   k_pair<T>    one wavefront per bin: full-shell gather over the stencil bins, canonical
-//                (lower-UID) orientation per pair so asymmetric terms match the reference
+//   k_binsort    one wavefront per bucket (element type x spatial bin): bitonic sort of the
+//                bucket's particle ids (deterministic order), gather of bin-local coordinates
+//   k_pair<T>    one thread per sorted particle: full-shell gather over the per-type-pair
+//                stencil, walked as contiguous z-runs; canonical (lower-UID) orientation per
+//                pair so asymmetric terms match the reference
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,6 +29,7 @@
 using namespace cil;
 
 #define WAVE 64
+#define GUARD_SLOTS 4
 
 // ======================================================================================
 // Bin keys
@@ -52,7 +54,7 @@ __global__ void k_keys(const StepParams* __restrict__ Pp, Dev S) {
     atomicOr(S.err, (unsigned)ERR_NONFINITE);
     for (int d = 0; d < P.box.dim; ++d) x[d] = 0.0;
   }
-  int b = bin_of(P, x);
+  int b = bucket_key(P, p, s, x);
   S.key[p] = b;
   S.rank[p] = atomicAdd(&S.count[b], 1);
 }
@@ -188,11 +190,12 @@ __global__ void __launch_bounds__(WAVE* BINSORT_WAVES) k_binsort(const StepParam
   const StepParams& P = *Pp;
   __shared__ int buf[BINSORT_WAVES][BINSORT_MAX];
   int wid = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
-  int b = blockIdx.x * BINSORT_WAVES + wid;
-  if (b >= P.nbins) return;
-  int s0 = S.start[b], n = S.start[b + 1] - s0;
-  if (lane == 0) S.count[b] = 0;  // reset for the next step's counting pass
+  int u = blockIdx.x * BINSORT_WAVES + wid;  // bucket (type, spatial bin)
+  if (u >= P.nbuckets) return;
+  int s0 = S.start[u], n = S.start[u + 1] - s0;
+  if (lane == 0) S.count[u] = 0;  // reset for the next step's counting pass
   if (n == 0) return;
+  const int b = u % P.nbins;
   int* L = buf[wid];
   if (n <= WAVE) {
     // register bitonic sort across the wave
@@ -213,8 +216,11 @@ __global__ void __launch_bounds__(WAVE* BINSORT_WAVES) k_binsort(const StepParam
       int p = v;
       int cslot = p / P.nE, e = p - cslot * P.nE;
       int sp = species_of_cell(P, cslot);
-      S.smeta[s0 + lane] = p * 8 + (sp * P.nE + e);
+      int t = sp * P.nE + e;
+      S.smeta[s0 + lane] = p * 8 + t;
+      S.sbin[s0 + lane] = b;
       S.inv[p] = s0 + lane;
+      if (S.sigma[p] > P.sigma_bound[t]) atomicOr(S.err, (unsigned)ERR_CUTOFF);
       write_sorted<T>(P, S, b, s0 + lane, p);
     }
     return;
@@ -250,15 +256,22 @@ __global__ void __launch_bounds__(WAVE* BINSORT_WAVES) k_binsort(const StepParam
     S.idx[s0 + i] = p;
     int cslot = p / P.nE, e = p - cslot * P.nE;
     int sp = species_of_cell(P, cslot);
-    S.smeta[s0 + i] = p * 8 + (sp * P.nE + e);
+    int t = sp * P.nE + e;
+    S.smeta[s0 + i] = p * 8 + t;
+    S.sbin[s0 + i] = b;
     S.inv[p] = s0 + i;
+    if (S.sigma[p] > P.sigma_bound[t]) atomicOr(S.err, (unsigned)ERR_CUTOFF);
     write_sorted<T>(P, S, b, s0 + i, p);
   }
 }
 
 // ======================================================================================
-// Pair forces: one wavefront per bin, lanes = particles of the bin, uniform loop over
-// the stencil bins' particles (neighbour data is wave-uniform -> scalar/broadcast loads).
+// Pair forces.  Thread per sorted particle: a wavefront holds 64 consecutive particles of
+// one element type, i.e. a short run of bins along z.  For each source type the stencil
+// (reach per type pair) is walked as (x, y) columns; the z-run of a column is one contiguous
+// range of the sorted array, walked in a single loop that only tracks bin boundaries for
+// the periodic shift.  Law parameters of the (target, source) type pair sit in registers;
+// every candidate gets a conservative cut-off test, then the exact reference predicate.
 // ======================================================================================
 __device__ inline void bin_coords(const StepParams& P, int b, int* bc) {
   if (P.box.dim == 3) {
@@ -273,15 +286,53 @@ __device__ inline void bin_coords(const StepParams& P, int b, int* bc) {
   }
 }
 
+// Law of one (target p, source q) element-type pair with both orientations' asymmetric
+// terms: the canonical "i" of a pair is its lower particle index (lower UID).
+struct PairLaw {
+  int law;
+  double rcut2, eps, xi, e0, ew, hzk;
+  double ewell_p, ewell_q;  // adhesion constant of p's / q's element type
+  double ea_p, ea_q;        // EA modulus of p's / q's element type
+};
+
+__device__ inline PairLaw load_law(const StepParams& P, int tp, int tq) {
+  const LawEntry& A = P.law[tp][tq];
+  const LawEntry& B = P.law[tq][tp];
+  PairLaw L;
+  L.law = A.law;
+  L.rcut2 = A.rcut2;
+  L.eps = A.eps;
+  L.xi = A.xi;
+  L.e0 = A.e0;
+  L.ew = A.ew;
+  L.hzk = A.hzk;
+  L.ewell_p = A.ewell_i;
+  L.ewell_q = B.ewell_i;
+  L.ea_p = A.ea_ei;
+  L.ea_q = B.ea_ei;
+  return L;
+}
+
+// Random direction of an exactly coincident 2D contact pair (MD.cu:321-331), canonical i.
+__device__ inline void coincident_dir(const StepParams& P, int pidx_i, int pidx_j, uint32_t ts, double* rij) {
+  double u[2];
+  int ci = pidx_i / P.nE, cj = pidx_j / P.nE;
+  int ei = pidx_i - ci * P.nE, ej = pidx_j - cj * P.nE;
+  int si_ = species_of_cell(P, ci), sj_ = species_of_cell(P, cj);
+  uint32_t ki = (cell_key(si_, ci - P.base[si_]) << 1) | ei, kj = (cell_key(sj_, cj - P.base[sj_]) << 1) | ej;
+  philox_uniform2(P.seed, ki, ts, ST_PAIR, kj, u);
+  double alpha = u[0] * kTwoPi;
+  rij[0] = sin(alpha);
+  rij[1] = cos(alpha);
+}
+
 // FP64 pair evaluation from p's side with the reference's operation order.
-// d = minimg(x_q - x_p); returns true when the pair interacts, adds to F and pr.
-__device__ inline void pair64(const StepParams& P, int pidx_p, int tp, int spec_p, double sp_, int pidx_q, int tq,
+// d = minimg(x_q - x_p) exactly as BoundingBox::distance; adds p's share to F and pr.
+__device__ inline void pair64(const StepParams& P, const PairLaw& E, int pidx_p, int spec_p, double sp_, int pidx_q,
                               int spec_q, double sq_, double d[3], double r2, uint32_t ts, double F[3], double* pr) {
   const int D = P.box.dim;
-  bool p_is_i = pidx_p < pidx_q;
-  int ti = p_is_i ? tp : tq, tj = p_is_i ? tq : tp;
+  const bool p_is_i = pidx_p < pidx_q;
   double si = p_is_i ? sp_ : sq_, sj = p_is_i ? sq_ : sp_;
-  const LawEntry& E = P.law[ti][tj];
   double f0;
   bool contact = false;
   double temp = 0.0;
@@ -301,17 +352,11 @@ __device__ inline void pair64(const StepParams& P, int pidx_p, int tp, int spec_
     double r = sqrt(r2);
     if (!(r <= (si + sj) / 2)) return;
     contact = true;
-    if (D == 2 && d[0] == 0.0 && d[1] == 0.0) {  // MD.cu:321-331, direction for canonical i
-      double u[2];
-      int ci = (p_is_i ? pidx_p : pidx_q) / P.nE, cj = (p_is_i ? pidx_q : pidx_p) / P.nE;
-      int ei = (p_is_i ? pidx_p : pidx_q) - ci * P.nE, ej = (p_is_i ? pidx_q : pidx_p) - cj * P.nE;
-      int si_ = species_of_cell(P, ci), sj_ = species_of_cell(P, cj);
-      uint32_t ki = (cell_key(si_, ci - P.base[si_]) << 1) | ei, kj = (cell_key(sj_, cj - P.base[sj_]) << 1) | ej;
-      philox_uniform2(P.seed, ki, ts, ST_PAIR, kj, u);
-      double alpha = u[0] * kTwoPi;
-      double rij0 = sin(alpha), rij1 = cos(alpha);
-      d[0] = p_is_i ? rij0 : -rij0;
-      d[1] = p_is_i ? rij1 : -rij1;
+    if (D == 2 && d[0] == 0.0 && d[1] == 0.0) {
+      double rij[2];
+      coincident_dir(P, p_is_i ? pidx_p : pidx_q, p_is_i ? pidx_q : pidx_p, ts, rij);
+      d[0] = p_is_i ? rij[0] : -rij[0];
+      d[1] = p_is_i ? rij[1] : -rij[1];
     }
     double hh = (si + sj) / 2 - r;
     double h = hh >= 0.0 ? hh : 0.0;
@@ -319,10 +364,10 @@ __device__ inline void pair64(const StepParams& P, int pidx_p, int tp, int spec_
       double Ri = si / 2, Rj = sj / 2;
       f0 = -(pow(h, 1.5) / (E.hzk * (sqrt((1 / Ri) + (1 / Rj)))));
     } else {
-      f0 = -(h * E.ea_ei);
+      f0 = -(h * (p_is_i ? E.ea_p : E.ea_q));
     }
     double A = D == 3 ? contact_area(r, si, sj) : contact_length(si, sj, h);
-    f0 += E.ewell_i * A;
+    f0 += (p_is_i ? E.ewell_p : E.ewell_q) * A;
     temp = (A == 0 ? 0 : (4 * kPi * sq(si / 2) / 3));
     temp = (D == 3 ? temp : A);
   } else {
@@ -348,26 +393,27 @@ __device__ inline bool lj_inside_exact(const StepParams& P, const Dev& S, int pi
   return r2 <= sij2 * rcut2;
 }
 
-__device__ inline void pair32(const StepParams& P, const Dev& S, int pidx_p, int tp, int spec_p, float sp_,
-                              int pidx_q, int tq, int spec_q, float sq_, float d[3], float r2, uint32_t ts, float F[3],
+// FP32 pair evaluation (fast mode).  Cut-off decisions of the discontinuous LJ law that fall
+// inside a 1e-5 relative guard band are re-decided in FP64 from the canonical positions.
+__device__ inline void pair32(const StepParams& P, const Dev& S, const PairLaw& E, int pidx_p, int spec_p, float sp_,
+                              int pidx_q, int spec_q, float sq_, float d[3], float r2, uint32_t ts, float F[3],
                               float* pr) {
   const int D = P.box.dim;
-  bool p_is_i = pidx_p < pidx_q;
-  int ti = p_is_i ? tp : tq, tj = p_is_i ? tq : tp;
-  const LawEntry& E = P.law[ti][tj];
+  const bool p_is_i = pidx_p < pidx_q;
   float f0;
   bool contact = false;
   float temp = 0.0f;
   if (E.law == CIL_LAW_LJ) {
-    float rcut2 = (float)E.rcut2;
     float sh = 0.5f * (sp_ + sq_);
     float sij2 = sh * sh;
-    float rc2 = sij2 * rcut2;
+    float rc2 = sij2 * (float)E.rcut2;
     if (r2 > rc2 * 1.00001f) return;
-    if (r2 > rc2 * 0.99999f) {
-      if (!lj_inside_exact(P, S, pidx_p, pidx_q, E.rcut2)) return;
-    }
-    float f = lj_f0(sij2, r2) - (P.force_shifted ? lj_f0(sij2, rc2) * sqrtf(rc2 / r2) : 0.0f);
+    if (r2 > rc2 * 0.99999f && !lj_inside_exact(P, S, pidx_p, pidx_q, E.rcut2)) return;
+    float ir2 = 1.0f / r2;
+    float s2 = sij2 * ir2;
+    float s6 = s2 * s2 * s2;
+    float f = (s6 - 2.0f * s6 * s6) * ir2;
+    if (P.force_shifted) f -= lj_f0(sij2, rc2) * sqrtf(rc2 * ir2);
     f0 = f * (float)E.eps;
   } else if (E.law == CIL_LAW_SC) {
     float xi = (float)E.xi, rcut2 = (float)E.rcut2;
@@ -383,16 +429,10 @@ __device__ inline void pair32(const StepParams& P, const Dev& S, int pidx_p, int
     if (!(r <= (si + sj) * 0.5f)) return;
     contact = true;
     if (D == 2 && d[0] == 0.0f && d[1] == 0.0f) {
-      double u[2];
-      int ci = (p_is_i ? pidx_p : pidx_q) / P.nE, cj = (p_is_i ? pidx_q : pidx_p) / P.nE;
-      int ei = (p_is_i ? pidx_p : pidx_q) - ci * P.nE, ej = (p_is_i ? pidx_q : pidx_p) - cj * P.nE;
-      int si_ = species_of_cell(P, ci), sj_ = species_of_cell(P, cj);
-      uint32_t ki = (cell_key(si_, ci - P.base[si_]) << 1) | ei, kj = (cell_key(sj_, cj - P.base[sj_]) << 1) | ej;
-      philox_uniform2(P.seed, ki, ts, ST_PAIR, kj, u);
-      double alpha = u[0] * kTwoPi;
-      float rij0 = (float)sin(alpha), rij1 = (float)cos(alpha);
-      d[0] = p_is_i ? rij0 : -rij0;
-      d[1] = p_is_i ? rij1 : -rij1;
+      double rij[2];
+      coincident_dir(P, p_is_i ? pidx_p : pidx_q, p_is_i ? pidx_q : pidx_p, ts, rij);
+      d[0] = (float)(p_is_i ? rij[0] : -rij[0]);
+      d[1] = (float)(p_is_i ? rij[1] : -rij[1]);
     }
     float hh = (si + sj) * 0.5f - r;
     float h = hh >= 0.0f ? hh : 0.0f;
@@ -400,21 +440,21 @@ __device__ inline void pair32(const StepParams& P, const Dev& S, int pidx_p, int
       float Ri = si * 0.5f, Rj = sj * 0.5f;
       f0 = -(h * sqrtf(h) / ((float)E.hzk * sqrtf(1.0f / Ri + 1.0f / Rj)));
     } else {
-      f0 = -(h * (float)E.ea_ei);
+      f0 = -(h * (float)(p_is_i ? E.ea_p : E.ea_q));
     }
     float A;
+    float ra = si * 0.5f, rb = sj * 0.5f;
     if (D == 3) {
-      float ra = si * 0.5f, rb = sj * 0.5f;
-      float t = sqrtf(4.f * r2 * ra * ra - (r2 - rb * rb + ra * ra) * (r2 - rb * rb + ra * ra)) / (2.f * r);
+      float u = r2 - rb * rb + ra * ra;
+      float t = sqrtf(4.f * r2 * ra * ra - u * u) / (2.f * r);
       A = t > 0.f ? (float)kPi * t * t : 0.f;
     } else {
-      float ra = si * 0.5f, rb = sj * 0.5f;
       float q = ra - (2.f * rb * h - h * h) / (2.f * (ra + rb - h));
       float t = 2.f * sqrtf(ra * ra - q * q);
       A = isnan(t) ? (si <= sj ? si : sj) : t;
     }
-    f0 += (float)E.ewell_i * A;
-    temp = (A == 0.f ? 0.f : (4.f * (float)kPi * (si * 0.5f) * (si * 0.5f) / 3.f));
+    f0 += (float)(p_is_i ? E.ewell_p : E.ewell_q) * A;
+    temp = (A == 0.f ? 0.f : (4.f * (float)kPi * ra * ra / 3.f));
     temp = (D == 3 ? temp : A);
   } else {
     return;
@@ -425,91 +465,125 @@ __device__ inline void pair32(const StepParams& P, const Dev& S, int pidx_p, int
   if (contact && P.pressure && temp != 0.0f) *pr -= fs / temp;
 }
 
+// Bucket of (element type t, spatial bin coordinates).
+__device__ inline int bucket_of(const StepParams& P, int t, int x, int y, int z) {
+  int b = P.box.dim == 3 ? (x * P.ns[1] + y) * P.ns[2] + z : x * P.ns[1] + y;
+  return t * P.nbins + b;
+}
+
+// Up to two z-segments of the stencil column [bz-R, bz+R] (split at the periodic seam).
+// Segment k covers bins z0[k]..z1[k] whose unwrapped offset from bz starts at off[k].
+__device__ inline int z_segments(const StepParams& P, int bz, int R, int* z0, int* z1, int* off) {
+  if (P.box.dim != 3) {
+    z0[0] = 0;
+    z1[0] = 0;
+    off[0] = 0;
+    return 1;
+  }
+  const int nz = P.ns[2];
+  int lo = bz - R, hi = bz + R;
+  if (!P.box.pbc[2]) {
+    lo = lo < 0 ? 0 : lo;
+    hi = hi > nz - 1 ? nz - 1 : hi;
+    z0[0] = lo;
+    z1[0] = hi;
+    off[0] = lo - bz;
+    return 1;
+  }
+  if (lo < 0) {
+    z0[0] = lo + nz; z1[0] = nz - 1; off[0] = lo - bz;
+    z0[1] = 0; z1[1] = hi; off[1] = -bz;
+    return 2;
+  }
+  if (hi > nz - 1) {
+    z0[0] = lo; z1[0] = nz - 1; off[0] = lo - bz;
+    z0[1] = 0; z1[1] = hi - nz; off[1] = nz - bz;
+    return 2;
+  }
+  z0[0] = lo; z1[0] = hi; off[0] = lo - bz;
+  return 1;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_pair(const StepParams* __restrict__ Pp, Dev S) {
   const StepParams& P = *Pp;
   const int D = P.box.dim;
-  int wid = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
-  int b = blockIdx.x * (blockDim.x / WAVE) + wid;
-  if (b >= P.nbins) return;
-  b = __builtin_amdgcn_readfirstlane(b);
-  const int s0 = S.start[b], n = S.start[b + 1] - s0;
-  if (n == 0) return;
-  int bc[3];
-  bin_coords(P, b, bc);
+  const int total = S.start[P.nbuckets];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total) return;
   const uint32_t ts = (uint32_t)*S.ts;
-  const T* spos = (const T*)S.spos;
-  for (int chunk = 0; chunk < n; chunk += WAVE) {
-    int k = s0 + chunk + lane;
-    bool valid = chunk + lane < n;
-    int kk = valid ? k : s0;
-    int meta = S.smeta[kk];
-    int pidx = meta >> 3, tp = meta & 7;
-    int cell_p = pidx / P.nE;
-    int spec_p = species_of_cell(P, cell_p);
-    T xp[4];
-    if constexpr (sizeof(T) == 4) {
-      float4 r = ((const float4*)spos)[kk];
-      xp[0] = r.x; xp[1] = r.y; xp[2] = r.z; xp[3] = r.w;
-    } else {
-      double4 r = ((const double4*)spos)[kk];
-      xp[0] = r.x; xp[1] = r.y; xp[2] = r.z; xp[3] = r.w;
-    }
-    T F[3] = {0, 0, 0};
-    T pr = 0;
-    const int rx = P.reach[0], ry = P.reach[1], rz = D == 3 ? P.reach[2] : 0;
-    for (int ox = -rx; ox <= rx; ++ox) {
+  const int meta = S.smeta[k];
+  const int pidx = meta >> 3, tp = meta & 7;
+  const int cshift = P.nE - 1;  // cell = particle >> cshift (nE is 1 or 2)
+  const int cell_p = pidx >> cshift;
+  const int spec_p = species_of_cell(P, cell_p);
+  int bc[3];
+  bin_coords(P, S.sbin[k], bc);
+  T xp[4];
+  if constexpr (sizeof(T) == 4) {
+    float4 r = ((const float4*)S.spos)[k];
+    xp[0] = r.x; xp[1] = r.y; xp[2] = r.z; xp[3] = r.w;
+  } else {
+    double4 r = ((const double4*)S.spos)[k];
+    xp[0] = r.x; xp[1] = r.y; xp[2] = r.z; xp[3] = r.w;
+  }
+  T F[3] = {0, 0, 0};
+  T pr = 0;
+  for (int tq = 0; tq < P.ntypes; ++tq) {
+    const int Rx = P.reach_tt[tp][tq][0];
+    if (Rx < 0) continue;
+    const int Ry = P.reach_tt[tp][tq][1], Rz = D == 3 ? P.reach_tt[tp][tq][2] : 0;
+    const PairLaw E = load_law(P, tp, tq);
+    const T rca = (T)P.rca_tt[tp][tq], rcb = (T)P.rcb_tt[tp][tq], margin = (T)(sizeof(T) == 4 ? 1.0001 : 1.000001);
+    int z0[2], z1[2], zo[2];
+    const int nseg = z_segments(P, bc[2], Rz, z0, z1, zo);
+    for (int ox = -Rx; ox <= Rx; ++ox) {
       int nx = bc[0] + ox;
-      T shx = 0;
-      if (P.box.pbc[0]) {
-        if (nx < 0) nx += P.ns[0];
-        else if (nx >= P.ns[0]) nx -= P.ns[0];
-      } else if (nx < 0 || nx >= P.ns[0]) continue;
-      shx = (T)(ox * P.w[0]);
-      for (int oy = -ry; oy <= ry; ++oy) {
+      if (P.box.pbc[0]) nx = nx < 0 ? nx + P.ns[0] : (nx >= P.ns[0] ? nx - P.ns[0] : nx);
+      else if (nx < 0 || nx >= P.ns[0]) continue;
+      for (int oy = -Ry; oy <= Ry; ++oy) {
         int ny = bc[1] + oy;
-        T shy = 0;
-        if (P.box.pbc[1]) {
-          if (ny < 0) ny += P.ns[1];
-          else if (ny >= P.ns[1]) ny -= P.ns[1];
-        } else if (ny < 0 || ny >= P.ns[1]) continue;
-        shy = (T)(oy * P.w[1]);
-        for (int oz = -rz; oz <= rz; ++oz) {
-          int nz = 0;
-          T shz = 0;
-          if (D == 3) {
-            nz = bc[2] + oz;
-            if (P.box.pbc[2]) {
-              if (nz < 0) nz += P.ns[2];
-              else if (nz >= P.ns[2]) nz -= P.ns[2];
-            } else if (nz < 0 || nz >= P.ns[2]) continue;
-            shz = (T)(oz * P.w[2]);
+        if (P.box.pbc[1]) ny = ny < 0 ? ny + P.ns[1] : (ny >= P.ns[1] ? ny - P.ns[1] : ny);
+        else if (ny < 0 || ny >= P.ns[1]) continue;
+        for (int sg = 0; sg < nseg; ++sg) {
+          const int u0 = bucket_of(P, tq, nx, ny, z0[sg]);
+          const int u1 = bucket_of(P, tq, nx, ny, z1[sg]);
+          const int j0 = S.start[u0], j1 = S.start[u1 + 1];
+          int u = u0, jend = S.start[u0 + 1];
+          int oz = zo[sg];
+          // p's position relative to the source bin's frame (FP32: bin-local coordinates)
+          T qx = 0, qy = 0, qz = 0;
+          if constexpr (sizeof(T) == 4) {
+            qx = xp[0] - (T)(ox * P.w[0]);
+            qy = xp[1] - (T)(oy * P.w[1]);
+            qz = xp[2] - (T)(oz * P.w[2]);
           }
-          int nb = D == 3 ? (nx * P.ns[1] + ny) * P.ns[2] + nz : nx * P.ns[1] + ny;
-          int j0 = S.start[nb], j1 = S.start[nb + 1];
           for (int j = j0; j < j1; ++j) {
-            int mq = S.smeta[j];
-            int qidx = mq >> 3, tq = mq & 7;
-            int cell_q = qidx / P.nE;
-            if (cell_q == cell_p) continue;
-            int spec_q = species_of_cell(P, cell_q);
+            while (j >= jend) {  // crossed into the next z-bin of the run
+              ++u;
+              ++oz;
+              jend = S.start[u + 1];
+              if constexpr (sizeof(T) == 4) qz -= (T)P.w[2];
+            }
+            const int mq = S.smeta[j];
+            const int qidx = mq >> 3;
             if constexpr (sizeof(T) == 4) {
-              float4 r = ((const float4*)spos)[j];
-              float d[3];
-              d[0] = (r.x + shx) - xp[0];
-              d[1] = (r.y + shy) - xp[1];
-              d[2] = D == 3 ? (r.z + shz) - xp[2] : 0.0f;
+              const float4 r = ((const float4*)S.spos)[j];
+              float d[3] = {r.x - qx, r.y - qy, D == 3 ? r.z - qz : 0.0f};
               if (P.reach[0] == 0 && P.box.pbc[0]) d[0] -= (float)P.box.L[0] * rintf(d[0] * (float)P.box.invL[0]);
               if (P.reach[1] == 0 && P.box.pbc[1]) d[1] -= (float)P.box.L[1] * rintf(d[1] * (float)P.box.invL[1]);
-              if (D == 3 && P.reach[2] == 0 && P.box.pbc[2]) d[2] -= (float)P.box.L[2] * rintf(d[2] * (float)P.box.invL[2]);
-              float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+              if (D == 3 && P.reach[2] == 0 && P.box.pbc[2])
+                d[2] -= (float)P.box.L[2] * rintf(d[2] * (float)P.box.invL[2]);
+              const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+              const float rc = fmaf(rca, r.w + xp[3], rcb);
+              if (r2 > rc * rc * margin || (qidx >> cshift) == cell_p) continue;
               float Fl[3] = {F[0], F[1], F[2]};
               float prl = pr;
-              pair32(P, S, pidx, tp, spec_p, xp[3], qidx, tq, spec_q, r.w, d, r2, ts, Fl, &prl);
+              pair32(P, S, E, pidx, spec_p, xp[3], qidx, species_of_cell(P, qidx >> cshift), r.w, d, r2, ts, Fl, &prl);
               F[0] = Fl[0]; F[1] = Fl[1]; F[2] = Fl[2];
               pr = prl;
             } else {
-              double4 r = ((const double4*)spos)[j];
+              const double4 r = ((const double4*)S.spos)[j];
               double xq[3] = {r.x, r.y, r.z};
               double d[3] = {0, 0, 0};
               double r2 = 0.0;
@@ -519,9 +593,12 @@ __global__ void __launch_bounds__(256) k_pair(const StepParams* __restrict__ Pp,
                 d[dd] = x;
                 r2 += sq(x);
               }
+              const double rc = rca * (r.w + (double)xp[3]) + rcb;
+              if (r2 > rc * rc * margin || (qidx >> cshift) == cell_p) continue;
               double Fl[3] = {(double)F[0], (double)F[1], (double)F[2]};
               double prl = pr;
-              pair64(P, pidx, tp, spec_p, (double)xp[3], qidx, tq, spec_q, r.w, d, r2, ts, Fl, &prl);
+              pair64(P, E, pidx, spec_p, (double)xp[3], qidx, species_of_cell(P, qidx >> cshift), r.w, d, r2, ts, Fl,
+                     &prl);
               F[0] = Fl[0]; F[1] = Fl[1]; F[2] = Fl[2];
               pr = prl;
             }
@@ -529,12 +606,233 @@ __global__ void __launch_bounds__(256) k_pair(const StepParams* __restrict__ Pp,
         }
       }
     }
-    if (valid) {
-      T* fp = (T*)S.fpair;
-      for (int dd = 0; dd < D; ++dd) fp[(size_t)dd * S.P + k] = F[dd];
-      ((T*)S.ppair)[k] = pr;
+  }
+  T* fp = (T*)S.fpair;
+  for (int dd = 0; dd < D; ++dd) fp[(size_t)dd * S.P + k] = F[dd];
+  ((T*)S.ppair)[k] = pr;
+}
+
+// Lean FP32 pair kernel (fast mode), specialised on the dimension.  Waves are laid out per
+// element type (the sorted array is type-major), so the target type, its species, the law
+// of each (target, source) type pair, the stencil reach and the cross-adhesion choice are
+// wave-uniform (scalar registers).  Same traversal and pair semantics as k_pair<float>.
+// Single-disc cells: the same-cell exclusion is j == k, so the inner loop reads only the
+// 16-byte sorted record of each candidate.
+template <int D>
+__global__ void __launch_bounds__(256) k_pair32(const StepParams* __restrict__ Pp, Dev S) {
+  const StepParams& P = *Pp;
+  const float4* __restrict__ spos = (const float4*)S.spos;
+  const int* __restrict__ smeta = S.smeta;
+  const int* __restrict__ start = S.start;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE);
+  // wave -> (type, chunk of that type's sorted range)
+  int tp = -1, k0 = 0, k1 = 0, acc = 0;
+  for (int t = 0; t < P.ntypes; ++t) {
+    const int T0 = start[t * P.nbins], T1 = start[(t + 1) * P.nbins];
+    const int nw = (T1 - T0 + WAVE - 1) / WAVE;
+    if (tp < 0 && w < acc + nw) {
+      tp = t;
+      k0 = T0 + (w - acc) * WAVE;
+      k1 = T1;
+    }
+    acc += nw;
+  }
+  if (tp < 0) return;
+  tp = __builtin_amdgcn_readfirstlane(tp);
+  const int k = k0 + lane;
+  if (k >= k1) return;
+  const uint32_t ts = (uint32_t)*S.ts;
+  const int nE = P.nE;
+  const int cshift = nE - 1;
+  const int spec_p = tp / nE;
+  const float ca_p = (float)P.cross_adh[spec_p];
+  const int pidx = smeta[k] >> 3;
+  const int cell_p = pidx >> cshift;
+  int bc[3];
+  bin_coords(P, S.sbin[k], bc);
+  const float4 me = spos[k];
+  const float wx = (float)P.w[0], wy = (float)P.w[1], wz = (float)P.w[2];
+  const bool mix = P.reach[0] == 0 && P.box.pbc[0], miy = P.reach[1] == 0 && P.box.pbc[1];
+  const bool miz = D == 3 && P.reach[2] == 0 && P.box.pbc[2];
+  const bool shifted = P.force_shifted != 0;
+  float Fx = 0.f, Fy = 0.f, Fz = 0.f, pr = 0.f;
+  int nguard = 0;
+  for (int tq = 0; tq < P.ntypes; ++tq) {
+    const int Rx = P.reach_tt[tp][tq][0];
+    if (Rx < 0) continue;
+    const int Ry = P.reach_tt[tp][tq][1], Rz = D == 3 ? P.reach_tt[tp][tq][2] : 0;
+    const LawEntry& A = P.law[tp][tq];
+    const LawEntry& B = P.law[tq][tp];
+    const int law = A.law;
+    const float rca = (float)P.rca_tt[tp][tq], rcb = (float)P.rcb_tt[tp][tq];
+    const float rcut2 = (float)A.rcut2, eps = (float)A.eps, xi = (float)A.xi, e0 = (float)A.e0, ew = (float)A.ew;
+    const float hzk = (float)A.hzk;
+    const float ewell_p = (float)A.ewell_i, ewell_q = (float)B.ewell_i, ea_p = (float)A.ea_ei, ea_q = (float)B.ea_ei;
+    const float cadh = (tq / nE == spec_p) ? 1.0f : ca_p;  // scale for f0 >= 0 (MD.cu:345-360)
+    const bool same_species_cells = nE == 2 && (tq / nE == spec_p);
+    const bool self_bucket = tq == tp;
+    int z0[2], z1[2], zo[2];
+    const int nseg = z_segments(P, bc[2], Rz, z0, z1, zo);
+    for (int ox = -Rx; ox <= Rx; ++ox) {
+      int nx = bc[0] + ox;
+      if (P.box.pbc[0]) nx = nx < 0 ? nx + P.ns[0] : (nx >= P.ns[0] ? nx - P.ns[0] : nx);
+      else if (nx < 0 || nx >= P.ns[0]) continue;
+      const float qx = me.x - (float)ox * wx;
+      for (int oy = -Ry; oy <= Ry; ++oy) {
+        int ny = bc[1] + oy;
+        if (P.box.pbc[1]) ny = ny < 0 ? ny + P.ns[1] : (ny >= P.ns[1] ? ny - P.ns[1] : ny);
+        else if (ny < 0 || ny >= P.ns[1]) continue;
+        const float qy = me.y - (float)oy * wy;
+        for (int sg = 0; sg < nseg; ++sg) {
+          const int u0 = bucket_of(P, tq, nx, ny, z0[sg]);
+          const int u1 = bucket_of(P, tq, nx, ny, z1[sg]);
+          const int j1 = start[u1 + 1];
+          int u = u0, jend = start[u0 + 1];
+          float qz = me.z - (float)zo[sg] * wz;
+          for (int j = start[u0]; j < j1; ++j) {
+            while (j >= jend) {
+              ++u;
+              jend = start[u + 1];
+              qz -= wz;
+            }
+            const float4 r = spos[j];
+            float dx = r.x - qx, dy = r.y - qy, dz = D == 3 ? r.z - qz : 0.0f;
+            if (mix) dx -= (float)P.box.L[0] * rintf(dx * (float)P.box.invL[0]);
+            if (miy) dy -= (float)P.box.L[1] * rintf(dy * (float)P.box.invL[1]);
+            if (miz) dz -= (float)P.box.L[2] * rintf(dz * (float)P.box.invL[2]);
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const float ssum = r.w + me.w;
+            const float rc = fmaf(rca, ssum, rcb);
+            if (r2 > rc * rc * 1.0001f) continue;
+            if (self_bucket && j == k) continue;
+            if (same_species_cells && ((smeta[j] >> 3) >> cshift) == cell_p) continue;
+            float f0;
+            bool contact = false;
+            float temp = 0.0f;
+            if (law == CIL_LAW_LJ) {  // LJParams::elemForce (MD.cuh:89-110)
+              const float sh = 0.5f * ssum;
+              const float sij2 = sh * sh;
+              const float rc2 = sij2 * rcut2;
+              if (r2 > rc2 * 1.00001f) continue;
+              if (r2 > rc2 * 0.99999f) {  // decided exactly in FP64 by k_lj_guard_fix
+                if (nguard < GUARD_SLOTS) S.gslot[(size_t)k * GUARD_SLOTS + nguard] = j;
+                ++nguard;
+                continue;
+              }
+              const float ir2 = 1.0f / r2;
+              const float s2 = sij2 * ir2;
+              const float s6 = s2 * s2 * s2;
+              float f = (s6 - 2.0f * s6 * s6) * ir2;
+              if (shifted) f -= lj_f0(sij2, rc2) * sqrtf(rc2 * ir2);
+              f0 = f * eps;
+            } else if (law == CIL_LAW_SC) {  // SCParams::elemForce (MD.cuh:190-207)
+              const float sigCore = 0.5f * ssum;
+              const float up = sigCore + xi, lo = sigCore - xi;
+              const float rc2 = rcut2 * up * up;
+              if (!(r2 > rcut2 * lo * lo && r2 <= rc2)) continue;
+              const float rr = sqrtf(r2);
+              const float y = rr - sigCore;
+              const float ixi3 = 6.0f / (xi * xi * xi);
+              if (y <= -xi) f0 = 0.0f;
+              else if (y < 0.0f) f0 = -(e0 + ew) * ((-y) * (y + xi) * ixi3) / rr;
+              else if (y < xi) f0 = ew * (y * (xi - y) * ixi3) / rr;
+              else f0 = 0.0f;
+              if (shifted) f0 -= sc_f0(sigCore, rc2, xi, e0, ew) * sqrtf(rc2 / r2);
+            } else {  // contact laws (device semantics MD.cu:317-362), canonical i = lower index
+              const int qidx = smeta[j] >> 3;
+              const bool p_is_i = pidx < qidx;
+              const float si = p_is_i ? me.w : r.w, sj = p_is_i ? r.w : me.w;
+              const float rr = sqrtf(r2);
+              if (!(rr <= (si + sj) * 0.5f)) continue;
+              contact = true;
+              if (D == 2 && dx == 0.0f && dy == 0.0f) {
+                double rij[2];
+                coincident_dir(P, p_is_i ? pidx : qidx, p_is_i ? qidx : pidx, ts, rij);
+                dx = (float)(p_is_i ? rij[0] : -rij[0]);
+                dy = (float)(p_is_i ? rij[1] : -rij[1]);
+              }
+              const float hh = (si + sj) * 0.5f - rr;
+              const float h = hh >= 0.0f ? hh : 0.0f;
+              const float ra = si * 0.5f, rb = sj * 0.5f;
+              if (law == CIL_LAW_HZ) f0 = -(h * sqrtf(h) / (hzk * sqrtf(1.0f / ra + 1.0f / rb)));
+              else f0 = -(h * (p_is_i ? ea_p : ea_q));
+              float Acon;
+              if (D == 3) {
+                const float uu = r2 - rb * rb + ra * ra;
+                const float t = sqrtf(4.f * r2 * ra * ra - uu * uu) / (2.f * rr);
+                Acon = t > 0.f ? (float)kPi * t * t : 0.f;
+              } else {
+                const float q = ra - (2.f * rb * h - h * h) / (2.f * (ra + rb - h));
+                const float t = 2.f * sqrtf(ra * ra - q * q);
+                Acon = isnan(t) ? (si <= sj ? si : sj) : t;
+              }
+              f0 += (p_is_i ? ewell_p : ewell_q) * Acon;
+              temp = (Acon == 0.f ? 0.f : (4.f * (float)kPi * ra * ra / 3.f));
+              temp = (D == 3 ? temp : Acon);
+            }
+            const float fs = f0 < 0.0f ? f0 : cadh * f0;
+            Fx = fmaf(fs, dx, Fx);
+            Fy = fmaf(fs, dy, Fy);
+            if (D == 3) Fz = fmaf(fs, dz, Fz);
+            if (contact && temp != 0.0f) pr -= fs / temp;
+          }
+        }
+      }
     }
   }
+  float* fp = (float*)S.fpair;
+  fp[k] = Fx;
+  fp[(size_t)S.P + k] = Fy;
+  if (D == 3) fp[2 * (size_t)S.P + k] = Fz;
+  ((float*)S.ppair)[k] = P.pressure ? pr : 0.0f;
+  S.gcount[k] = nguard;
+  if (nguard > GUARD_SLOTS) atomicOr(S.err, (unsigned)ERR_GUARD);
+}
+
+// LJ pairs whose FP32 distance fell within 1e-5 of the cut-off: exact FP64 decision
+// (LJParams::elemForce predicate r^2 <= RCUT^2 sigma_ij^2 on the canonical positions), then
+// the FP32 force added by the thread that owns the target particle (deterministic).
+__global__ void k_lj_guard_fix(const StepParams* __restrict__ Pp, Dev S) {
+  const StepParams& P = *Pp;
+  const int total = S.start[P.nbuckets];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total) return;
+  const int n = S.gcount[k];
+  if (n == 0) return;
+  const int D = P.box.dim;
+  const int meta = S.smeta[k];
+  const int pidx = meta >> 3, tp = meta & 7;
+  float* fp = (float*)S.fpair;
+  float F[3] = {0.f, 0.f, 0.f};
+  for (int g = 0; g < n && g < GUARD_SLOTS; ++g) {
+    const int j = S.gslot[(size_t)k * GUARD_SLOTS + g];
+    const int mq = S.smeta[j];
+    const int qidx = mq >> 3, tq = mq & 7;
+    const LawEntry& A = P.law[tp][tq];
+    if (!lj_inside_exact(P, S, pidx, qidx, A.rcut2)) continue;
+    double xp[3], xq[3], d64[3];
+    for (int d = 0; d < D; ++d) {
+      xp[d] = S.pos[(size_t)d * S.P + pidx];
+      xq[d] = S.pos[(size_t)d * S.P + qidx];
+    }
+    box_dist(P.box, d64, xp, xq);
+    float dx[3] = {(float)d64[0], (float)d64[1], D == 3 ? (float)d64[2] : 0.0f};
+    const float r2 = fmaf(dx[0], dx[0], fmaf(dx[1], dx[1], dx[2] * dx[2]));
+    const float sh = 0.5f * (((const float4*)S.spos)[k].w + ((const float4*)S.spos)[j].w);
+    const float sij2 = sh * sh;
+    const float rc2 = sij2 * (float)A.rcut2;
+    const float ir2 = 1.0f / r2;
+    const float s2 = sij2 * ir2;
+    const float s6 = s2 * s2 * s2;
+    float f = (s6 - 2.0f * s6 * s6) * ir2;
+    if (P.force_shifted) f -= lj_f0(sij2, rc2) * sqrtf(rc2 * ir2);
+    float f0 = f * (float)A.eps;
+    const int spec_p = tp / P.nE, spec_q = tq / P.nE;
+    const float fs = (f0 < 0.0f || spec_p == spec_q) ? f0 : (float)P.cross_adh[spec_p] * f0;
+    for (int d = 0; d < D; ++d) F[d] = fmaf(fs, dx[d], F[d]);
+  }
+  for (int d = 0; d < D; ++d) fp[(size_t)d * S.P + k] += F[d];
 }
 
 // ======================================================================================
@@ -710,7 +1008,7 @@ __global__ void __launch_bounds__(256) k_update(const StepParams* __restrict__ P
         atomicOr(S.err, (unsigned)ERR_NONFINITE);
         for (int d = 0; d < D; ++d) xn[d] = 0.0;
       }
-      int b = bin_of(P, xn);
+      int b = bucket_key(P, (int)p, s, xn);
       S.key[p] = b;
       S.rank[p] = atomicAdd(&S.count[b], 1);
     }
@@ -762,7 +1060,7 @@ __global__ void k_pairs_list(const StepParams* __restrict__ Pp, Dev S, int reach
   const StepParams& P = *Pp;
   const int D = P.box.dim;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int total = S.start[P.nbins];
+  int total = S.start[P.nbuckets];
   if (k >= total) return;
   int meta = S.smeta[k];
   int pidx = meta >> 3, tp = meta & 7;
@@ -771,65 +1069,63 @@ __global__ void k_pairs_list(const StepParams* __restrict__ Pp, Dev S, int reach
   for (int d = 0; d < D; ++d) xp[d] = S.pos[(size_t)d * S.P + pidx];
   double sp_ = S.sigma[pidx];
   int bc[3];
-  bc[0] = bin_coord(P, 0, xp[0]);
-  bc[1] = bin_coord(P, 1, xp[1]);
-  bc[2] = D == 3 ? bin_coord(P, 2, xp[2]) : 0;
+  bin_coords(P, S.sbin[k], bc);
   int reach[3] = {reach_x, reach_y, D == 3 ? reach_z : 0};
-  for (int ox = -reach[0]; ox <= reach[0]; ++ox)
-    for (int oy = -reach[1]; oy <= reach[1]; ++oy)
-      for (int oz = -reach[2]; oz <= reach[2]; ++oz) {
-        int o[3] = {ox, oy, oz}, nc[3] = {0, 0, 0};
-        bool ok = true;
-        for (int d = 0; d < D; ++d) {
-          int v = bc[d] + o[d];
-          if (P.box.pbc[d]) v = ((v % P.ns[d]) + P.ns[d]) % P.ns[d];
-          else if (v < 0 || v >= P.ns[d]) ok = false;
-          nc[d] = v;
-        }
-        if (!ok) continue;
-        // skip duplicate visits of a bin when the stencil wraps around a small periodic grid
-        bool dup = false;
-        for (int d = 0; d < D && !dup; ++d) {
-          if (P.box.pbc[d] && 2 * reach[d] + 1 > P.ns[d]) {
-            int lo = -reach[d];
-            // canonical offset: smallest offset in [-reach, reach] reaching the same bin
-            for (int t = lo; t < o[d]; ++t)
-              if (((bc[d] + t) % P.ns[d] + P.ns[d]) % P.ns[d] == nc[d]) dup = true;
+  for (int tq = 0; tq < P.ntypes; ++tq) {
+    const LawEntry& E = P.law[tp][tq];
+    if (E.law == CIL_LAW_NONE) continue;
+    for (int ox = -reach[0]; ox <= reach[0]; ++ox)
+      for (int oy = -reach[1]; oy <= reach[1]; ++oy)
+        for (int oz = -reach[2]; oz <= reach[2]; ++oz) {
+          int o[3] = {ox, oy, oz}, nc[3] = {0, 0, 0};
+          bool ok = true;
+          for (int d = 0; d < D; ++d) {
+            int v = bc[d] + o[d];
+            if (P.box.pbc[d]) v = ((v % P.ns[d]) + P.ns[d]) % P.ns[d];
+            else if (v < 0 || v >= P.ns[d]) ok = false;
+            nc[d] = v;
           }
-        }
-        if (dup) continue;
-        int nb = D == 3 ? (nc[0] * P.ns[1] + nc[1]) * P.ns[2] + nc[2] : nc[0] * P.ns[1] + nc[1];
-        for (int j = S.start[nb]; j < S.start[nb + 1]; ++j) {
-          int qidx = S.smeta[j] >> 3, tq = S.smeta[j] & 7;
-          if (qidx <= pidx) continue;
-          if (qidx / P.nE == cell_p) continue;
-          int ti = tp, tj = tq;  // p is canonical i (lower slot = lower UID)
-          const LawEntry& E = P.law[ti][tj];
-          if (E.law == CIL_LAW_NONE) continue;
-          double xq[3], rij[3];
-          for (int d = 0; d < D; ++d) xq[d] = S.pos[(size_t)d * S.P + qidx];
-          double r2 = box_dist(P.box, rij, xp, xq);
-          double sq_ = S.sigma[qidx];
-          bool hit = false;
-          if (E.law == CIL_LAW_LJ) {
-            double sij2 = sq(0.5 * (sp_ + sq_));
-            hit = r2 <= sij2 * E.rcut2;
-          } else if (E.law == CIL_LAW_SC) {
-            double sigCore = 0.5 * (sp_ + sq_);
-            hit = r2 > E.rcut2 * sq(sigCore - E.xi) && r2 <= E.rcut2 * sq(sigCore + E.xi);
-          } else {
-            hit = sqrt(r2) <= (sp_ + sq_) / 2;
+          if (!ok) continue;
+          // skip duplicate visits of a bin when the stencil wraps around a small periodic grid
+          bool dup = false;
+          for (int d = 0; d < D && !dup; ++d) {
+            if (P.box.pbc[d] && 2 * reach[d] + 1 > P.ns[d]) {
+              for (int t = -reach[d]; t < o[d]; ++t)
+                if (((bc[d] + t) % P.ns[d] + P.ns[d]) % P.ns[d] == nc[d]) dup = true;
+            }
           }
-          if (hit) {
-            unsigned long long slot = atomicAdd(counter, 1ull);
-            if ((long)slot < cap) {
-              out[2 * slot] = slot_to_uid_particle(P, S.ncells, pidx);
-              out[2 * slot + 1] = slot_to_uid_particle(P, S.ncells, qidx);
+          if (dup) continue;
+          int nb = bucket_of(P, tq, nc[0], nc[1], nc[2]);
+          for (int j = S.start[nb]; j < S.start[nb + 1]; ++j) {
+            int qidx = S.smeta[j] >> 3;
+            if (qidx <= pidx) continue;
+            if (qidx / P.nE == cell_p) continue;
+            double xq[3], rij[3];
+            for (int d = 0; d < D; ++d) xq[d] = S.pos[(size_t)d * S.P + qidx];
+            double r2 = box_dist(P.box, rij, xp, xq);
+            double sq_ = S.sigma[qidx];
+            bool hit = false;
+            if (E.law == CIL_LAW_LJ) {
+              double sij2 = sq(0.5 * (sp_ + sq_));
+              hit = r2 <= sij2 * E.rcut2;
+            } else if (E.law == CIL_LAW_SC) {
+              double sigCore = 0.5 * (sp_ + sq_);
+              hit = r2 > E.rcut2 * sq(sigCore - E.xi) && r2 <= E.rcut2 * sq(sigCore + E.xi);
+            } else {
+              hit = sqrt(r2) <= (sp_ + sq_) / 2;
+            }
+            if (hit) {
+              unsigned long long slot = atomicAdd(counter, 1ull);
+              if ((long)slot < cap) {
+                out[2 * slot] = slot_to_uid_particle(P, S.ncells, pidx);
+                out[2 * slot + 1] = slot_to_uid_particle(P, S.ncells, qidx);
+              }
             }
           }
         }
-      }
+  }
 }
 
 #include "cil_division.inc"
+#include "cil_surrogate.inc"
 #include "cil_context.inc"

@@ -21,7 +21,8 @@ enum CellStateV : int { CS_DIVIDING = 0, CS_CRAWLING = 1, CS_UNDEFINED = 2 };  /
 
 // Error bits raised by kernels and collected by the host at synchronisation points.
 enum ErrBit : unsigned { ERR_BIN_OVERFLOW = 1u, ERR_CAPACITY = 2u, ERR_BOND = 4u, ERR_NONFINITE = 8u,
-                         ERR_CUTOFF = 16u, ERR_NOISE = 32u };
+                         ERR_CUTOFF = 16u, ERR_NOISE = 32u, ERR_GUARD = 64u,
+                         ERR_KNN = 128u };
 
 // ---------------------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11).  Counter = (id, ts, stream, k), key = seed.
@@ -156,13 +157,18 @@ struct StepParams {
   double max_disp;
   cil_wall walls[CIL_MAX_WALLS];
   cil_species sp[CIL_MAX_SPECIES];
-  // bins
+  // bins: spatial grid ns[0] x ns[1] x ns[2]; particles are counting-sorted into buckets
+  // (element type, spatial bin), bucket = type * nbins + bin
   int ns[3];
   double w[3];      // bin widths (L/ns)
   double invw[3];
   int nbins;
-  int reach[3];     // stencil half-width per dim (0 when ns==1)
-  double rc_guard2; // (max cut-off)^2 check bound
+  int nbuckets;
+  int reach[3];     // largest stencil half-width per dim (0 when ns==1)
+  int reach_tt[CIL_MAX_TYPES][CIL_MAX_TYPES][3];  // per (target, source) type; -1: no law
+  double rca_tt[CIL_MAX_TYPES][CIL_MAX_TYPES];    // force cut-off rc = rca*(s_p+s_q) + rcb
+  double rcb_tt[CIL_MAX_TYPES][CIL_MAX_TYPES];
+  double sigma_bound[CIL_MAX_TYPES];              // diameters the bins were sized for
 };
 
 __device__ inline int species_of_cell(const StepParams& P, int cslot) {

@@ -47,6 +47,9 @@ struct Dev {
   int* inv;
   void* spos;
   int* smeta;
+  int* sbin;        // spatial bin of each sorted particle
+  int* gslot;       // [P][4] FP32 LJ pairs inside the cut-off guard band (sorted source index)
+  int* gcount;      // [P] number of guard-band pairs per sorted particle
   void* fpair;
   void* ppair;
   const double* ext;
@@ -82,6 +85,12 @@ __device__ inline int bin_of(const StepParams& P, const double* x) {
   int b = bin_coord(P, 0, x[0]) * P.ns[1] + bin_coord(P, 1, x[1]);
   if (P.box.dim == 3) b = b * P.ns[2] + bin_coord(P, 2, x[2]);
   return b;
+}
+
+// Counting-sort key of particle p (slot index) at position x: element type major.
+__device__ inline int bucket_key(const StepParams& P, int p, int s, const double* x) {
+  int t = s * P.nE + (p % P.nE);
+  return t * P.nbins + bin_of(P, x);
 }
 
 // ---------------------------------------------------------------------------------------

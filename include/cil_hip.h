@@ -239,7 +239,16 @@ int cil_hip_bin_info(cil_ctx* ctx, int* nbins, double* width);
  * pair kernel (64, 128 or 256; 0 keeps the current value). */
 int cil_hip_set_tuning(cil_ctx* ctx, int bins_per_cutoff, int block);
 
-/* Per-kernel-family HIP-event timing of eager steps (project addition; off by default). */
+/* FP32 pair-kernel variant (project addition, autotuner knob): 0 = type-specialised lean
+ * kernel (default), 1 = generic kernel. */
+int cil_hip_set_kernel(cil_ctx* ctx, int variant);
+
+/* Autotuner (project addition): times `reps` force passes for every bins-per-cut-off /
+ * pair-block / kernel-variant candidate, keeps the fastest and reports it.  Needs forces
+ * computed by cil_hip_forces or a step (the state is not advanced). */
+int cil_hip_autotune(cil_ctx* ctx, int reps, int* best_bins, int* best_block, int* best_variant, double* best_us);
+
+/* Per-kernel-family HIP-event timing (project addition; off by default). */
 int cil_hip_set_timing(cil_ctx* ctx, int on);
 
 /* sizeof the ABI structs (0: cil_params, 1: cil_species, 2: cil_species_view, 3: cil_box),

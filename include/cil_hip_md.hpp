@@ -1,0 +1,167 @@
+// cil_hip_md.hpp — C++ host mirror of the reference's step interfaces over the C ABI.
+//
+// The reference drives the step through process-global state (CIL.cu:13-23): a CellColony
+// (Colony.hpp:16-209), the MDIntegrator function pointer `mdStep` (MD.cuh:450-452) called
+// GTS times per frame (CIL.cu:1106), calculForcesPressure at start-up (CIL.cu:1096) and
+// checkPositions once per frame (CIL.cu:1107).  This header gives the same shapes:
+//
+//   cil::hip::CellColony     per-species arrays in the reference layout, addCell/removeCell
+//   cil::hip::Session        owns one device context (upload, step, download)
+//   cil::hip::EulerHIP       `void (*)(int& ts)`-compatible step on the bound session
+//
+// Errors surface as std::runtime_error, the reference's run_assert convention
+// (Utils.hpp:99-109).  Host arrays are never cached by the device side.
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cil_hip.h"
+
+namespace cil {
+namespace hip {
+
+struct SpeciesArrays {
+  int ncells = 0;
+  std::vector<double> pos, frc, vel, sigma;                       // (cap, nelem[, dim])
+  std::vector<double> sigma_t, pressure, theta, rho, ang_vel, rest_len;  // (cap)
+  std::vector<int> state, t0, tswitch, tpressure;                 // (cap)
+};
+
+// Per-species SoA store with the reference's row-major layout pos(cell, elem, d)
+// (Array.h:557-600) and its append / swap-with-last semantics (Colony.cpp:355-399).
+class CellColony {
+ public:
+  CellColony(int dim, int nelem, const std::vector<int>& capacity) : dim_(dim), nelem_(nelem) {
+    for (int cap : capacity) {
+      SpeciesArrays s;
+      size_t c = (size_t)cap;
+      s.pos.assign(c * nelem * dim, 0.0);
+      s.frc.assign(c * nelem * dim, 0.0);
+      s.vel.assign(c * nelem * dim, 0.0);
+      s.sigma.assign(c * nelem, 1.0);
+      for (auto* v : {&s.sigma_t, &s.pressure, &s.theta, &s.rho, &s.ang_vel, &s.rest_len}) v->assign(c, 0.0);
+      s.state.assign(c, 2);
+      s.t0.assign(c, 0);
+      s.tswitch.assign(c, -1);
+      s.tpressure.assign(c, 0);
+      spec_.push_back(std::move(s));
+      cap_.push_back(cap);
+    }
+  }
+  int getCellTypes() const { return (int)spec_.size(); }
+  int getNumCells(int s) const { return spec_[s].ncells; }
+  int getTotalCells() const {
+    int n = 0;
+    for (auto& s : spec_) n += s.ncells;
+    return n;
+  }
+  int capacity(int s) const { return cap_[s]; }
+  int dim() const { return dim_; }
+  int nelem() const { return nelem_; }
+  double& pos(int s, int c, int e, int d) { return spec_[s].pos[((size_t)c * nelem_ + e) * dim_ + d]; }
+  double& frc(int s, int c, int e, int d) { return spec_[s].frc[((size_t)c * nelem_ + e) * dim_ + d]; }
+  double& sigma(int s, int c, int e) { return spec_[s].sigma[(size_t)c * nelem_ + e]; }
+  SpeciesArrays& species(int s) { return spec_[s]; }
+
+  int addCell(int s) {  // Colony.cpp:355-359: append, returns the new index
+    if (spec_[s].ncells >= cap_[s]) throw std::runtime_error("CellColony::addCell: capacity exhausted");
+    return spec_[s].ncells++;
+  }
+
+  std::vector<cil_species_view> views() {
+    std::vector<cil_species_view> v(spec_.size());
+    for (size_t k = 0; k < spec_.size(); ++k) {
+      SpeciesArrays& s = spec_[k];
+      cil_species_view& w = v[k];
+      w.ncells = s.ncells;
+      w.capacity = cap_[k];
+      w.pos = s.pos.data();
+      w.frc = s.frc.data();
+      w.vel = s.vel.data();
+      w.sigma = s.sigma.data();
+      w.sigma_t = s.sigma_t.data();
+      w.pressure = s.pressure.data();
+      w.state = s.state.data();
+      w.t0 = s.t0.data();
+      w.tswitch = s.tswitch.data();
+      w.tpressure = s.tpressure.data();
+      w.theta = s.theta.data();
+      w.rho = s.rho.data();
+      w.ang_vel = s.ang_vel.data();
+      w.rest_len = s.rest_len.data();
+    }
+    return v;
+  }
+  void pull_counts(const std::vector<cil_species_view>& v) {
+    for (size_t k = 0; k < spec_.size(); ++k) spec_[k].ncells = v[k].ncells;
+  }
+
+ private:
+  int dim_, nelem_;
+  std::vector<SpeciesArrays> spec_;
+  std::vector<int> cap_;
+};
+
+inline void run_assert(int rc, cil_ctx* ctx, const char* what) {
+  if (rc != CIL_OK)
+    throw std::runtime_error(std::string(what) + ": " + (ctx ? cil_hip_last_error(ctx) : "") + " (status " +
+                             std::to_string(rc) + ")");
+}
+
+// One device context bound to a host colony.  Device state is resident between calls;
+// the host arrays are only touched by upload() and download().
+class Session {
+ public:
+  Session(const cil_params& p, const cil_box& box, CellColony& col) : col_(col) {
+    std::vector<int> cap(CIL_MAX_SPECIES, 0);
+    for (int s = 0; s < col.getCellTypes(); ++s) cap[s] = col.capacity(s);
+    run_assert(cil_hip_create(&p, &box, cap.data(), &ctx_), nullptr, "cil_hip_create");
+  }
+  ~Session() { cil_hip_destroy(ctx_); }
+  Session(const Session&) = delete;
+  Session& operator=(const Session&) = delete;
+
+  void upload() {
+    auto v = col_.views();
+    run_assert(cil_hip_upload(ctx_, v.data(), (int)v.size()), ctx_, "cil_hip_upload");
+  }
+  // calculForcesPressure (CIL.cu:999-1007), as called once before the frame loop (CIL.cu:1096)
+  void calculForcesPressure() { run_assert(cil_hip_forces(ctx_, nullptr, nullptr), ctx_, "cil_hip_forces"); }
+  // `gts` Euler steps (the frame's inner loop, CIL.cu:1106)
+  void steps(int gts, int& ts) { run_assert(cil_hip_step(ctx_, gts, &ts), ctx_, "cil_hip_step"); }
+  // checkPositions (CIL.cu:945-966); returns the number of drained cells
+  int checkPositions() {
+    int removed = 0;
+    run_assert(cil_hip_check_positions(ctx_, &removed), ctx_, "checkPositions");
+    return removed;
+  }
+  void download() {
+    auto v = col_.views();
+    std::vector<int> need(v.size());
+    run_assert(cil_hip_download(ctx_, v.data(), (int)v.size(), need.data()), ctx_, "cil_hip_download");
+    col_.pull_counts(v);
+  }
+  cil_ctx* ctx() { return ctx_; }
+
+ private:
+  CellColony& col_;
+  cil_ctx* ctx_ = nullptr;
+};
+
+// The MDIntegrator-shaped entry point: bind a session once, then assign
+// `mdStep = cil::hip::EulerHIP;` in place of Euler (Input.cpp:526-532).
+inline Session*& bound_session() {
+  static Session* s = nullptr;
+  return s;
+}
+inline void bind(Session* s) { bound_session() = s; }
+inline void EulerHIP(int& ts) {
+  Session* s = bound_session();
+  if (!s) throw std::runtime_error("EulerHIP: no session bound");
+  s->steps(1, ts);
+}
+
+}  // namespace hip
+}  // namespace cil

@@ -1,0 +1,112 @@
+"""CPU oracle of the drift surrogate (TEST INFRASTRUCTURE ONLY, numpy FP64).
+
+Restates what the HIP surrogate kernels compute, so they can be checked:
+  - features: for each single-disc particle, its 16 nearest neighbours (minimum image,
+    ties broken by particle index) in ascending distance, each as (dx, dy, dz, s_j) with
+    s_j = +sigma_j for species 0 and -sigma_j for species 1; then sigma_i, species_i, the
+    cell axis (3 zeros for single discs) and 3 zero pads -> 72 features;
+  - MLP 72 -> 128 -> 128 -> 3, SiLU after the hidden layers, output scaled by OUT_SCALE;
+  - loss: mean over samples and components of (y - t)^2 / 2 on the scaled target;
+  - backward and an Adam step (Kingma & Ba 2015).
+The surrogate has no reference counterpart: parity of these functions is pinned by finite
+differences (backward) and by the GPU kernels agreeing with them, not by reference vectors.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+NNB = 16
+NFEAT = 72
+HIDDEN = 128
+NOUT = 3
+OUT_SCALE = 0.01
+
+
+def knn_features(pos, sigma, species, box_len, k=NNB):
+    """Features of single-disc particles in a periodic cubic-or-rectangular 3D box."""
+    from scipy.spatial import cKDTree
+
+    L = np.asarray(box_len, dtype=float)
+    x = np.mod(pos, L)
+    tree = cKDTree(x, boxsize=L)
+    # ask for a few extra neighbours so distance ties can be ordered by index
+    d, j = tree.query(x, k=k + 4)
+    n = len(x)
+    feats = np.zeros((n, NFEAT))
+    for i in range(n):
+        dj = d[i][1:]
+        jj = j[i][1:]
+        order = np.lexsort((jj, dj))[:k]
+        jj = jj[order]
+        dx = x[jj] - x[i]
+        dx -= L * np.round(dx / L)
+        s = np.where(species[jj] == 0, sigma[jj], -sigma[jj])
+        feats[i, : 4 * k] = np.concatenate([dx, s[:, None]], axis=1).reshape(-1)
+        feats[i, 64] = sigma[i]
+        feats[i, 65] = species[i]
+    return feats
+
+
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def dsilu(z):
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s * (1.0 + z * (1.0 - s))
+
+
+def init_params(seed=0):
+    rng = np.random.default_rng(seed)
+    p = {
+        "W1": rng.normal(0, np.sqrt(2.0 / NFEAT), (HIDDEN, NFEAT)),
+        "b1": np.zeros(HIDDEN),
+        "W2": rng.normal(0, np.sqrt(2.0 / HIDDEN), (HIDDEN, HIDDEN)),
+        "b2": np.zeros(HIDDEN),
+        "W3": rng.normal(0, np.sqrt(1.0 / HIDDEN), (NOUT, HIDDEN)),
+        "b3": np.zeros(NOUT),
+    }
+    return p
+
+
+def forward(p, X):
+    z1 = X @ p["W1"].T + p["b1"]
+    h1 = silu(z1)
+    z2 = h1 @ p["W2"].T + p["b2"]
+    h2 = silu(z2)
+    y = h2 @ p["W3"].T + p["b3"]
+    return y, (X, z1, h1, z2, h2)
+
+
+def loss_and_grad(p, X, T):
+    """T: targets already divided by OUT_SCALE.  Returns loss and gradients."""
+    y, (X, z1, h1, z2, h2) = forward(p, X)
+    n = X.shape[0]
+    dy = (y - T) / (n * NOUT)
+    loss = 0.5 * np.sum((y - T) ** 2) / (n * NOUT)
+    g = {}
+    g["W3"] = dy.T @ h2
+    g["b3"] = dy.sum(0)
+    dh2 = dy @ p["W3"]
+    dz2 = dh2 * dsilu(z2)
+    g["W2"] = dz2.T @ h1
+    g["b2"] = dz2.sum(0)
+    dh1 = dz2 @ p["W2"]
+    dz1 = dh1 * dsilu(z1)
+    g["W1"] = dz1.T @ X
+    g["b1"] = dz1.sum(0)
+    return loss, g
+
+
+def adam_step(p, g, state, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    t = state.get("t", 0) + 1
+    state["t"] = t
+    for k in p:
+        m = state.setdefault("m_" + k, np.zeros_like(p[k]))
+        v = state.setdefault("v_" + k, np.zeros_like(p[k]))
+        m[:] = b1 * m + (1 - b1) * g[k]
+        v[:] = b2 * v + (1 - b2) * g[k] ** 2
+        mh = m / (1 - b1 ** t)
+        vh = v / (1 - b2 ** t)
+        p[k] = p[k] - lr * mh / (np.sqrt(vh) + eps)
+    return p

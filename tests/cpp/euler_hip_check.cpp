@@ -1,0 +1,105 @@
+// C++ drop-in check: drives the GPU step through the reference-shaped C++ layer
+// (cil::hip::CellColony, Session, EulerHIP as an MDIntegrator) and compares the result
+// with the CPU oracle on the same input.  Exit codes: 0 ok, 1 mismatch, 3 no GPU device.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+
+#include "../../include/cil_hip_md.hpp"
+#include "../../oracle/cil_oracle.h"
+
+typedef void (*MDIntegrator)(int& ts);  // MD.cuh:450
+
+int main() {
+  const int n = 150, nsteps = 200;
+  const double L = 18.0;
+  cil_params p;
+  std::memset(&p, 0, sizeof p);
+  p.abi_version = CIL_ABI_VERSION;
+  p.dim = 2;
+  p.nspecies = 1;
+  cil_species& sp = p.species[0];
+  sp.nelem = 2;
+  for (int e = 0; e < 2; ++e) {
+    sp.sigma[e] = 1.0;
+    sp.zeta[e] = 1.0;
+    sp.eps_core[e] = 1.0;
+    sp.sig_well[e] = 1.0;
+  }
+  sp.motility[1] = 0.1;
+  sp.spring = CIL_SPRING_HARMONIC;
+  sp.k_harm = 10.0;
+  sp.cross_adhesion = 1.0;
+  sp.cycle_p0 = 1.0;
+  sp.kT = 0.01;
+  for (int i = 0; i < CIL_MAX_TYPES; ++i)
+    for (int j = 0; j < CIL_MAX_TYPES; ++j) {
+      p.law[i][j] = (i < 2 && j < 2) ? CIL_LAW_SC : CIL_LAW_NONE;
+      p.rcut2[i][j] = 1.0;
+    }
+  p.dt = 1e-4;
+  p.noise = CIL_NOISE_FDT;
+  p.seed = 7;
+  p.precision = CIL_PREC_FP64;
+  cil_box box;
+  std::memset(&box, 0, sizeof box);
+  box.length[0] = box.length[1] = L;
+  box.periodic[0] = box.periodic[1] = 1;
+
+  cil::hip::CellColony gpu(2, 2, {n}), cpu(2, 2, {n});
+  std::mt19937_64 rng(5);
+  std::uniform_real_distribution<double> U(0.0, 1.0);
+  for (int c = 0; c < n; ++c) {
+    int a = gpu.addCell(0);
+    cpu.addCell(0);
+    double cx = U(rng) * L, cy = U(rng) * L, th = U(rng) * 6.283185307179586;
+    double ux = std::cos(th), uy = std::sin(th);
+    double xb[2] = {std::fmod(cx - 0.25 * ux + L, L), std::fmod(cy - 0.25 * uy + L, L)};
+    double xf[2] = {std::fmod(cx + 0.25 * ux + L, L), std::fmod(cy + 0.25 * uy + L, L)};
+    for (int d = 0; d < 2; ++d) {
+      gpu.pos(0, a, 0, d) = cpu.pos(0, a, 0, d) = xb[d];
+      gpu.pos(0, a, 1, d) = cpu.pos(0, a, 1, d) = xf[d];
+    }
+    for (int e = 0; e < 2; ++e) gpu.sigma(0, a, e) = cpu.sigma(0, a, e) = 0.9 + 0.2 * U(rng);
+    gpu.species(0).rest_len[a] = cpu.species(0).rest_len[a] = 0.5;
+    gpu.species(0).state[a] = cpu.species(0).state[a] = 1;
+  }
+  try {
+    cil::hip::Session session(p, box, gpu);
+    session.upload();
+    session.calculForcesPressure();  // CIL.cu:1096
+    cil::hip::bind(&session);
+    MDIntegrator mdStep = cil::hip::EulerHIP;
+    int ts = 0;
+    for (int k = 0; k < nsteps; ++k) mdStep(ts);
+    session.download();
+    // oracle on the host copy
+    or_ws* w = or_create(&p, &box);
+    or_set_traversal(w, 1);
+    auto v = cpu.views();
+    int tso = 0;
+    if (or_forces(w, v.data(), 1, 0) || or_step(w, v.data(), 1, nsteps, &tso)) {
+      std::printf("oracle failed: %s\n", or_last_error(w));
+      return 1;
+    }
+    or_destroy(w);
+    double err = 0.0;
+    for (int c = 0; c < n; ++c)
+      for (int e = 0; e < 2; ++e)
+        for (int d = 0; d < 2; ++d) {
+          double x = gpu.pos(0, c, e, d) - cpu.pos(0, c, e, d);
+          x -= L * std::round(x / L);
+          err = std::fmax(err, std::fabs(x));
+        }
+    std::printf("ts=%d max position difference %.3e\n", ts, err);
+    return (ts == nsteps && err <= 1e-9) ? 0 : 1;
+  } catch (const std::runtime_error& e) {
+    if (std::strstr(e.what(), "status -2")) {
+      std::printf("no GPU device: %s\n", e.what());
+      return 3;
+    }
+    std::printf("error: %s\n", e.what());
+    return 1;
+  }
+}
