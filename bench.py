@@ -39,15 +39,7 @@ def log(*a):
 
 
 def relaxed_state(n, seed=1):
-    p, box, col = configs.bulk_3d_mixed(n=n, seed=seed, precision=abi.PREC_FP32)
-    q = configs.relaxation_params(p)
-    md = HipMD(q, box, [sp["capacity"] for sp in col.species])
-    md.upload(col)
-    md.forces(want=False)
-    md.step(100, 0)
-    md.download(col)
-    md.close()
-    return p, box, col
+    return configs.relaxed_bulk_3d(n, seed=seed, precision=abi.PREC_FP32)
 
 
 def cpu_baseline(seed=1, n_sample=1 << 17):
@@ -79,6 +71,61 @@ def cpu_baseline(seed=1, n_sample=1 << 17):
             "value_1core": res[1][0]}
 
 
+MLP_FLOP_PER_PARTICLE = 2 * (72 * 128 + 128 * 128 + 128 * 3)  # 51,968 useful FLOP per inference
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def surrogate_leg(md, ts, n, plain_value, args):
+    """Train the drift surrogate on the GPU from plain trajectories of the same state, then
+    time the surrogate-assisted step (judge every 16th macro-step, fallback above tol)."""
+    t0 = time.perf_counter()
+    stride = max(1, n // (1 << 17))
+    nmacro = max(1, (2 << 20) // ((n + stride - 1) // stride))
+    ts, nsamp = md.surrogate_collect(nmacro, stride, 2 << 20, ts)
+    log(f"surrogate: {nsamp} samples from {nmacro} plain macro-steps in {time.perf_counter() - t0:.1f} s")
+    md.mlp_init(1)
+    t0 = time.perf_counter()
+    k1 = int(args.train_steps * 0.7)
+    losses = np.concatenate([md.surrogate_train(k1, batch=8192, lr=1e-3, seed=1),
+                             md.surrogate_train(args.train_steps - k1, batch=8192, lr=2e-4, seed=2)])
+    t_train = time.perf_counter() - t0
+    log(f"surrogate: {args.train_steps} Adam steps in {t_train:.1f} s, loss {losses[0]:.3e} -> "
+        f"{losses[-50:].mean():.3e}")
+    ts, rms, mx = md.surrogate_eval(ts)
+    log(f"surrogate: 8-step error rms/max network {rms[0]:.3e}/{mx[0]:.3e} zero-drift {rms[1]:.3e}/{mx[1]:.3e} "
+        f"euler-8dt {rms[2]:.3e}/{mx[2]:.3e}")
+    # per-kernel averages (timing mode synchronises after every macro-step)
+    md.set_timing(True)
+    ts, _ = md.surrogate_step(8, ts, judge_every=0, tol=args.tol)
+    kus = md.timings()
+    md.set_timing(False)
+    md.surrogate_stats_reset()
+    nmac = max(args.judge_every, (args.steps + 7) // 8)
+    ts, _ = md.surrogate_step(2, ts, judge_every=0, tol=args.tol)  # warm-up
+    md.num_cells()
+    md.surrogate_stats_reset()
+    t0 = time.perf_counter()
+    ts, st = md.surrogate_step(nmac, ts, judge_every=args.judge_every, tol=args.tol)
+    md.num_cells()
+    el = time.perf_counter() - t0
+    value = n * 8 * nmac / el
+    log(f"surrogate: {nmac} macro-steps in {el * 1e3:.1f} ms -> {value:.3e} particle-steps/s "
+        f"({value / plain_value:.2f}x plain); stats {st}")
+    mlp_us = kus.get("mlp", 0.0)
+    mfma_tf = MLP_FLOP_PER_PARTICLE * n / (mlp_us * 1e-6) / 1e12 if mlp_us > 0 else 0.0
+    return {
+        "value": value, "unit": "particle-steps/s", "speedup_vs_plain": value / plain_value,
+        "macro_steps": nmac, "judge_every": args.judge_every, "tol_sigma": args.tol, "stats": st,
+        "eval_err_rms": {"network": rms[0], "zero_drift": rms[1], "euler_8dt": rms[2]},
+        "eval_err_max": {"network": mx[0], "zero_drift": mx[1], "euler_8dt": mx[2]},
+        "training": {"samples": nsamp, "adam_steps": args.train_steps, "batch": 8192, "seconds": t_train,
+                     "loss_first": float(losses[0]), "loss_last50": float(losses[-50:].mean())},
+        "kernels_us": {k: kus[k] for k in ("sort", "knn", "mlp", "macro") if k in kus},
+        "mfma": {"kernel": "k_mlp_forward", "achieved": mfma_tf, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": mfma_tf / MFMA_BF16_PEAK_TFLOPS, "flop_per_particle": MLP_FLOP_PER_PARTICLE},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +134,10 @@ def main():
     ap.add_argument("--n", type=int, default=N_PARTICLES)
     ap.add_argument("--bins-per-cutoff", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--judge-every", type=int, default=16)
+    ap.add_argument("--tol", type=float, default=1e-3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the multi-GPU "
@@ -139,6 +190,8 @@ def main():
                      "bytes_per_particle": PAIR_BYTES_PER_PARTICLE},
         "step_hbm_gbs": STEP_BYTES_PER_PARTICLE * value / 1e9,
     }
+    if not args.no_surrogate:
+        out["surrogate"] = surrogate_leg(md, ts, n, value, args)
     md.close()
     if not args.no_cpu:
         try:

@@ -110,6 +110,17 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcil_hip.
 _lib = None
 
 
+class SurrogateStats(C.Structure):
+    _fields_ = [
+        ("macro_steps", C.c_long), ("plain_steps", C.c_long), ("judged", C.c_long), ("failed", C.c_long),
+        ("last_err_rms", C.c_double), ("last_err_max", C.c_double), ("worst_err_rms", C.c_double),
+    ]
+
+
+MLP_NPARAM = 26243
+MLP_NFEAT = 72
+
+
 class CilError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"cil_hip error {code}: {msg}")
@@ -141,6 +152,22 @@ def lib():
     L.cil_hip_set_timing.argtypes = [vp, C.c_int]
     L.cil_hip_set_kernel.argtypes = [vp, C.c_int]
     L.cil_hip_autotune.argtypes = [vp, C.c_int, PI, PI, PI, PD]
+    PF = C.POINTER(C.c_float)
+    PL = C.POINTER(C.c_long)
+    L.cil_hip_mlp_nparams.argtypes = []
+    L.cil_hip_mlp_set_params.argtypes = [vp, PF, C.c_size_t]
+    L.cil_hip_mlp_get_params.argtypes = [vp, PF, C.c_size_t]
+    L.cil_hip_mlp_init.argtypes = [vp, C.c_uint64]
+    L.cil_hip_mlp_forward.argtypes = [vp, PF, C.c_size_t, PF]
+    L.cil_hip_mlp_grad.argtypes = [vp, PF, PF, C.c_size_t, PF, PF]
+    L.cil_hip_knn_features.argtypes = [vp, PF, C.c_size_t]
+    L.cil_hip_surrogate_collect.argtypes = [vp, C.c_int, C.c_int, C.c_long, PI, PL]
+    L.cil_hip_surrogate_set_samples.argtypes = [vp, PF, PF, C.c_long]
+    L.cil_hip_surrogate_clear_samples.argtypes = [vp]
+    L.cil_hip_surrogate_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_uint64, PF]
+    L.cil_hip_surrogate_eval.argtypes = [vp, PI, PD, PD]
+    L.cil_hip_surrogate_step.argtypes = [vp, C.c_int, C.c_int, C.c_double, PI, C.POINTER(SurrogateStats)]
+    L.cil_hip_surrogate_stats_reset.argtypes = [vp]
     L.cil_hip_last_error.argtypes = [vp]
     L.cil_hip_last_error.restype = C.c_char_p
     L.cil_hip_version.restype = C.c_char_p
@@ -165,4 +192,8 @@ EXPORTS = (
     "cil_hip_forces", "cil_hip_pairs", "cil_hip_check_positions", "cil_hip_set_external_noise",
     "cil_hip_num_cells", "cil_hip_timings", "cil_hip_bin_info", "cil_hip_set_tuning", "cil_hip_set_timing", "cil_hip_set_kernel", "cil_hip_autotune",
     "cil_hip_sizeof", "cil_hip_last_error", "cil_hip_version",
+    "cil_hip_mlp_nparams", "cil_hip_mlp_set_params", "cil_hip_mlp_get_params", "cil_hip_mlp_init",
+    "cil_hip_mlp_forward", "cil_hip_mlp_grad", "cil_hip_knn_features", "cil_hip_surrogate_collect",
+    "cil_hip_surrogate_set_samples", "cil_hip_surrogate_clear_samples", "cil_hip_surrogate_train", "cil_hip_surrogate_eval",
+    "cil_hip_surrogate_step", "cil_hip_surrogate_stats_reset",
 )

@@ -208,3 +208,85 @@ class HipMD:
         _check(self.L, self.ctx, self.L.cil_hip_timings(self.ctx, us, 16, C.byref(n), names, 256), "timings")
         keys = names.value.decode().split(";")
         return {keys[k]: us[k] for k in range(n.value)}
+
+    # ---------------------------------------------------------------- drift surrogate
+    def _f32(self, a, shape=None):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if shape is not None and a.shape != shape:
+            raise ValueError(f"expected shape {shape}, got {a.shape}")
+        return a
+
+    def mlp_set_params(self, flat):
+        p = self._f32(flat, (abi.MLP_NPARAM,))
+        _check(self.L, self.ctx, self.L.cil_hip_mlp_set_params(self.ctx, _fp(p), p.size), "mlp_set_params")
+
+    def mlp_get_params(self):
+        p = np.zeros(abi.MLP_NPARAM, dtype=np.float32)
+        _check(self.L, self.ctx, self.L.cil_hip_mlp_get_params(self.ctx, _fp(p), p.size), "mlp_get_params")
+        return p
+
+    def mlp_init(self, seed=0):
+        _check(self.L, self.ctx, self.L.cil_hip_mlp_init(self.ctx, int(seed)), "mlp_init")
+
+    def mlp_forward(self, X):
+        X = self._f32(X)
+        n = X.shape[0]
+        Y = np.zeros((n, 3), dtype=np.float32)
+        _check(self.L, self.ctx, self.L.cil_hip_mlp_forward(self.ctx, _fp(X), n, _fp(Y)), "mlp_forward")
+        return Y
+
+    def mlp_grad(self, X, T):
+        X = self._f32(X)
+        T = self._f32(T)
+        g = np.zeros(abi.MLP_NPARAM, dtype=np.float32)
+        loss = C.c_float()
+        _check(self.L, self.ctx, self.L.cil_hip_mlp_grad(self.ctx, _fp(X), _fp(T), X.shape[0], _fp(g), C.byref(loss)),
+               "mlp_grad")
+        return g, float(loss.value)
+
+    def knn_features(self, nrows):
+        F = np.zeros((nrows, abi.MLP_NFEAT), dtype=np.float32)
+        _check(self.L, self.ctx, self.L.cil_hip_knn_features(self.ctx, _fp(F), nrows), "knn_features")
+        return F
+
+    def surrogate_collect(self, nmacro, stride, cap, ts):
+        t = C.c_int(ts)
+        n = C.c_long()
+        _check(self.L, self.ctx, self.L.cil_hip_surrogate_collect(self.ctx, int(nmacro), int(stride), int(cap),
+                                                                  C.byref(t), C.byref(n)), "surrogate_collect")
+        return t.value, n.value
+
+    def surrogate_set_samples(self, X, T):
+        X = self._f32(X)
+        T = self._f32(T)
+        _check(self.L, self.ctx, self.L.cil_hip_surrogate_set_samples(self.ctx, _fp(X), _fp(T), X.shape[0]),
+               "surrogate_set_samples")
+
+    def surrogate_train(self, nsteps, batch=4096, lr=1e-3, seed=0):
+        losses = np.zeros(max(nsteps, 1), dtype=np.float32)
+        _check(self.L, self.ctx, self.L.cil_hip_surrogate_train(self.ctx, int(nsteps), int(batch), float(lr), int(seed),
+                                                                _fp(losses)), "surrogate_train")
+        return losses[:nsteps]
+
+    def surrogate_eval(self, ts):
+        """One judged macro-step: RMS/max errors of (network, zero drift, one 8dt Euler step)."""
+        t = C.c_int(ts)
+        rms = (C.c_double * 3)()
+        mx = (C.c_double * 3)()
+        _check(self.L, self.ctx, self.L.cil_hip_surrogate_eval(self.ctx, C.byref(t), rms, mx), "surrogate_eval")
+        return t.value, list(rms), list(mx)
+
+    def surrogate_step(self, nmacro, ts, judge_every=16, tol=1e-3):
+        t = C.c_int(ts)
+        st = abi.SurrogateStats()
+        _check(self.L, self.ctx, self.L.cil_hip_surrogate_step(self.ctx, int(nmacro), int(judge_every), float(tol),
+                                                               C.byref(t), C.byref(st)), "surrogate_step")
+        stats = {k: getattr(st, k) for k, _ in abi.SurrogateStats._fields_}
+        return t.value, stats
+
+    def surrogate_stats_reset(self):
+        _check(self.L, self.ctx, self.L.cil_hip_surrogate_stats_reset(self.ctx), "surrogate_stats_reset")
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))

@@ -197,6 +197,24 @@ def relaxation_params(p: abi.Params, dt=1e-3, max_disp=0.05) -> abi.Params:
     return q
 
 
+def relaxed_bulk_3d(n, seed=1, steps=100, **kw):
+    """The 3D mixed bulk of `n` particles relaxed on the GPU with the capped relaxation above
+    (`steps` steps); returns production (params, box, colony)."""
+    from .colony import HipMD
+
+    p, box, col = bulk_3d_mixed(n=n, seed=seed, **kw)
+    q = relaxation_params(p)
+    md = HipMD(q, box, [sp["capacity"] for sp in col.species])
+    try:
+        md.upload(col)
+        md.forces(want=False)
+        md.step(steps, 0)
+        md.download(col)
+    finally:
+        md.close()
+    return p, box, col
+
+
 # ------------------------------------------------------------------------------------------
 # 2D channel: growing two-disc colony with Hertz contacts, walls and drain
 # ------------------------------------------------------------------------------------------

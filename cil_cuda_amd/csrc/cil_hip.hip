@@ -1129,3 +1129,4 @@ __global__ void k_pairs_list(const StepParams* __restrict__ Pp, Dev S, int reach
 #include "cil_division.inc"
 #include "cil_surrogate.inc"
 #include "cil_context.inc"
+#include "cil_surrogate_host.inc"

@@ -258,6 +258,68 @@ size_t cil_hip_sizeof(int which);
 const char* cil_hip_last_error(cil_ctx* ctx);
 const char* cil_hip_version(void);
 
+/* ---------------------------------------------------------------------------------------
+ * Drift surrogate (project addition; no counterpart in the reference).
+ *
+ * A per-particle MLP (72 -> 128 -> 128 -> 3, SiLU, BF16 weights and activations on MFMA,
+ * FP32 accumulation) predicts the deterministic displacement over 8 Euler steps from the
+ * 16 nearest neighbours: 16 x (dx, dy, dz, +-sigma_j) (+ for species 0), own sigma, own
+ * species, 6 zeros.  A macro-step adds that drift plus the exact sum of the 8 steps' FDT
+ * noise increments.  Supported: 3D periodic single-disc colonies in FP32 force mode, no
+ * division, walls, motility, pressure friction, drag or active noise; other set-ups return
+ * CIL_E_INVALID.  The kNN search visits shells of bins around each particle until the 16th
+ * neighbour is covered, up to 3 shells (fewer in boxes of < 7 bins per dimension); a 16th
+ * neighbour beyond that returns CIL_E_INVALID.
+ * ------------------------------------------------------------------------------------- */
+typedef struct cil_surrogate_stats {
+  long macro_steps;     /* macro-steps advanced by the network */
+  long plain_steps;     /* Euler steps run by the plain integrator (judging and fallback) */
+  long judged;          /* judged macro-steps */
+  long failed;          /* judged macro-steps whose RMS error exceeded the tolerance */
+  double last_err_rms;  /* RMS over particles of |x_network - x_plain| at the last judgement */
+  double last_err_max;
+  double worst_err_rms;
+} cil_surrogate_stats;
+
+/* Number of network parameters (W1[128][72], b1, W2[128][128], b2, W3[3][128], b3; row-major,
+ * out x in, in this order) = 26243. */
+int cil_hip_mlp_nparams(void);
+int cil_hip_mlp_set_params(cil_ctx* ctx, const float* params, size_t n);
+int cil_hip_mlp_get_params(cil_ctx* ctx, float* params, size_t n);
+/* He-normal weights, zero biases, from Philox keyed by seed; resets the Adam state. */
+int cil_hip_mlp_init(cil_ctx* ctx, uint64_t seed);
+/* Y (n, 3) = drift in length units for features X (n, 72), rounded to BF16 on entry. */
+int cil_hip_mlp_forward(cil_ctx* ctx, const float* X, size_t n, float* Y);
+/* Gradient of the loss mean((y - T / 0.01)^2) / 2 over the n x 3 outputs (y = network output
+ * before the 0.01 output scale; T in length units), no update.  n <= 65536. */
+int cil_hip_mlp_grad(cil_ctx* ctx, const float* X, const float* T, size_t n, float* grad, float* loss);
+/* Features of the current state, rows in UID order (species-major). */
+int cil_hip_knn_features(cil_ctx* ctx, float* F, size_t rows);
+/* Runs nmacro x 8 plain Euler steps from the current state (ts advances by 8 per macro-step)
+ * and appends, for every stride-th sorted particle of each macro-step start (rotating phase),
+ * its features and its 8-step displacement minus the exact noise sum to the device training
+ * set (capacity cap samples). */
+int cil_hip_surrogate_collect(cil_ctx* ctx, int nmacro, int stride, long cap, int* ts, long* nsamples);
+/* Replaces the training set with n host samples: X (n, 72) rounded to BF16, T (n, 3) drift
+ * in length units. */
+int cil_hip_surrogate_set_samples(cil_ctx* ctx, const float* X, const float* T, long n);
+int cil_hip_surrogate_clear_samples(cil_ctx* ctx);
+/* nsteps Adam steps (beta 0.9/0.999, eps 1e-8) on minibatches of `batch` samples drawn with
+ * replacement (Philox keyed by seed); losses[it] receives each minibatch loss (may be NULL). */
+int cil_hip_surrogate_train(cil_ctx* ctx, int nsteps, int batch, float lr, uint64_t seed, float* losses);
+/* One judged macro-step from the current state: the network, zero drift and one Euler step of
+ * 8 dt are each compared with 8 plain steps (whose result is kept; ts += 8).
+ * err_rms[3], err_max[3] in that order. */
+int cil_hip_surrogate_eval(cil_ctx* ctx, int* ts, double* err_rms, double* err_max);
+/* nmacro surrogate-assisted macro-steps (ts += 8 each).  Every judge_every-th macro-step is
+ * judged: the plain integrator runs it (its result is kept) and the network's prediction is
+ * compared; an RMS error above tol switches to the plain integrator for the next judge_every
+ * macro-steps, after which the next macro-step is judged again.  judge_every 0: no judging.
+ * st (may be NULL) receives the cumulative statistics. */
+int cil_hip_surrogate_step(cil_ctx* ctx, int nmacro, int judge_every, double tol, int* ts,
+                           cil_surrogate_stats* st);
+int cil_hip_surrogate_stats_reset(cil_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,270 @@
+"""GPU tests of the drift surrogate through the C ABI.
+
+Bars (BF16 operands with FP32 accumulation against the FP64 oracle on the same BF16-rounded
+weights and inputs):
+  - forward: |y_gpu - y_ref| <= 0.03 * rms(y_ref) + 1e-6 per output (the GPU also rounds
+    hidden activations to BF16: ~0.4 % per layer);
+  - gradient: per tensor ||g_gpu - g_ref|| <= 0.05 * ||g_ref||, loss within 2 %;
+  - Adam: one step moves every parameter whose reference gradient is not tiny by lr
+    (first Adam step with bias correction), to 1e-3 * lr, against the gradient's sign for
+    >= 99.9 % of them;
+  - kNN features: equal to the oracle's rows rounded to BF16 (one BF16 ulp) for every
+    particle without a near-tie (relative gap < 1e-4) among its 17 nearest distances;
+  - training on plain trajectories: the trained network's 8-step displacement error is
+    below the zero-drift error (what the network gains over the method without it), and
+    the numbers are printed next to one Euler step of 8 dt.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from cil_cuda_amd import abi, configs
+from cil_cuda_amd.colony import HipMD
+from oracle import mlp_oracle as M
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def bf16(x):
+    """Round to BF16 (nearest even) and back, as the library does on entry."""
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def flatten(p):
+    return np.concatenate([p[k].reshape(-1) for k in ("W1", "b1", "W2", "b2", "W3", "b3")]).astype(np.float32)
+
+
+def unflatten(v):
+    out, o = {}, 0
+    for k, shape in (("W1", (M.HIDDEN, M.NFEAT)), ("b1", (M.HIDDEN,)), ("W2", (M.HIDDEN, M.HIDDEN)),
+                     ("b2", (M.HIDDEN,)), ("W3", (M.NOUT, M.HIDDEN)), ("b3", (M.NOUT,))):
+        n = int(np.prod(shape))
+        out[k] = np.asarray(v[o:o + n], dtype=np.float64).reshape(shape)
+        o += n
+    assert o == abi.MLP_NPARAM
+    return out
+
+
+def rounded(p):
+    """BF16 weights (as the MFMA operands) with FP32 biases."""
+    q = {k: v.copy() for k, v in p.items()}
+    for k in ("W1", "W2", "W3"):
+        q[k] = bf16(q[k])
+    for k in ("b1", "b2", "b3"):
+        q[k] = q[k].astype(np.float32).astype(np.float64)
+    return q
+
+
+def small_ctx(n=3000, seed=5):
+    p, box, col = configs.bulk_3d_mixed(n=n, seed=seed, precision=abi.PREC_FP32)
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    return md, p, box, col
+
+
+def test_abi_exports_surrogate(gpu):
+    L = abi.lib()
+    assert L.cil_hip_mlp_nparams() == abi.MLP_NPARAM == 26243
+
+
+def random_params(seed):
+    rng = np.random.default_rng(seed)
+    p = M.init_params(seed)
+    for k in ("b1", "b2", "b3"):
+        p[k] = rng.normal(0, 0.1, p[k].shape)
+    return p
+
+
+def test_mlp_forward_matches_oracle(gpu):
+    md, *_ = small_ctx(600)
+    rng = np.random.default_rng(11)
+    p = random_params(3)
+    n = 1000  # not a multiple of the 32-row tile
+    X = rng.normal(0, 1, (n, M.NFEAT))
+    md.mlp_set_params(flatten(p))
+    y = md.mlp_forward(X).astype(np.float64)
+    yr, _ = M.forward(rounded(p), bf16(X))
+    yr *= M.OUT_SCALE
+    err = np.abs(y - yr).max()
+    scale = np.sqrt(np.mean(yr ** 2))
+    say(f"forward: max |dy| {err:.3e}, rms(y) {scale:.3e}")
+    assert err <= 0.03 * scale + 1e-6
+    # a planted transposition of W2 must fail the same bar (the bar separates layouts)
+    q = rounded(p)
+    q["W2"] = q["W2"].T.copy()
+    yt, _ = M.forward(q, bf16(X))
+    assert np.abs(y - M.OUT_SCALE * yt).max() > 0.03 * scale + 1e-6
+    # round trip of the parameters
+    assert np.array_equal(md.mlp_get_params(), flatten(p))
+    md.close()
+
+
+def test_mlp_grad_matches_oracle(gpu):
+    md, *_ = small_ctx(600)
+    rng = np.random.default_rng(12)
+    p = random_params(4)
+    n = 777
+    X = rng.normal(0, 1, (n, M.NFEAT))
+    T = rng.normal(0, 0.01, (n, 3))
+    md.mlp_set_params(flatten(p))
+    g, loss = md.mlp_grad(X, T)
+    lr, gr = M.loss_and_grad(rounded(p), bf16(X), T.astype(np.float32).astype(np.float64) / M.OUT_SCALE)
+    gg = unflatten(g)
+    say(f"loss gpu {loss:.6e} oracle {lr:.6e}")
+    assert abs(loss - lr) <= 0.02 * abs(lr)
+    for k in ("W1", "b1", "W2", "b2", "W3", "b3"):
+        rel = np.linalg.norm(gg[k] - gr[k]) / np.linalg.norm(gr[k])
+        say(f"grad {k}: rel err {rel:.3e}")
+        assert rel <= 0.05, k
+    md.close()
+
+
+def test_adam_first_step(gpu):
+    md, *_ = small_ctx(600)
+    rng = np.random.default_rng(13)
+    p = random_params(6)
+    x = rng.normal(0, 1, (1, M.NFEAT))
+    t = rng.normal(0, 0.01, (1, 3))
+    flat = flatten(p)
+    md.mlp_set_params(flat)
+    md.surrogate_set_samples(x, t)
+    lr = 1e-3
+    losses = md.surrogate_train(1, batch=64, lr=lr, seed=1)
+    after = md.mlp_get_params().astype(np.float64)
+    _, gr = M.loss_and_grad(rounded(p), bf16(x), t.astype(np.float32).astype(np.float64) / M.OUT_SCALE)
+    g = np.concatenate([gr[k].reshape(-1) for k in ("W1", "b1", "W2", "b2", "W3", "b3")])
+    big = np.abs(g) > 1e-3 * np.abs(g).max()
+    step = after - flat.astype(np.float64)
+    say(f"adam: {big.sum()} parameters with clear gradient sign, loss {losses[0]:.4e}")
+    assert big.sum() > 1000
+    tol = 1e-3 * lr + 1e-7 * np.abs(flat[big])
+    # magnitude: exactly lr for every clear gradient (independent of the BF16 gradient error)
+    assert np.all(np.abs(np.abs(step[big]) - lr) <= tol)
+    # direction: against the reference gradient's sign (BF16 may flip a rare near-cancelling one)
+    agree = np.abs(step[big] + lr * np.sign(g[big])) <= tol
+    say(f"adam: sign agreement {agree.mean():.5f}")
+    assert agree.mean() >= 0.999
+    md.close()
+
+
+def _near_tie(d):
+    d = np.sort(d)[:17]
+    gaps = np.diff(d) / np.maximum(d[1:], 1e-12)
+    return np.any(gaps < 1e-4)
+
+
+def test_knn_features_match_oracle(gpu):
+    md, p, box, col = small_ctx(3000, seed=7)
+    n = col.total_cells()
+    F = md.knn_features(n).astype(np.float64)
+    x = col.flat("pos").reshape(-1, 3)
+    sig = col.flat("sigma").reshape(-1)
+    spec = np.concatenate([np.full(sp["ncells"], s) for s, sp in enumerate(col.species)])
+    L = np.array(box.length[:3])
+    ref = M.knn_features(x, sig, spec, L)
+    xw = np.mod(x, L)
+    checked = 0
+    for i in range(n):
+        d = xw - xw[i]
+        d -= L * np.round(d / L)
+        r = np.sqrt((d * d).sum(1))
+        r[i] = np.inf
+        if _near_tie(r):
+            continue
+        checked += 1
+        want = bf16(ref[i])
+        tol = 2.0 ** -7 * np.abs(want) + 1e-6
+        if not np.all(np.abs(F[i] - want) <= tol):
+            bad = np.nonzero(np.abs(F[i] - want) > tol)[0]
+            raise AssertionError(f"row {i}: features {bad[:8]} gpu {F[i][bad[:8]]} oracle {want[bad[:8]]}")
+    say(f"knn: {checked}/{n} rows compared")
+    assert checked >= 0.95 * n
+    md.close()
+
+
+def test_surrogate_requires_supported_setup(gpu):
+    p, box, col = configs.plain_2d_sc(n_cells=64, L=16.0, seed=2)
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    with pytest.raises(abi.CilError, match="surrogate"):
+        md.knn_features(col.total_cells())
+    md.close()
+
+
+@pytest.fixture(scope="module")
+def trained():
+    """A 4096-particle relaxed bulk, 24 plain macro-steps of training data, 800 Adam steps."""
+    p, box, col = configs.relaxed_bulk_3d(4096, seed=3, precision=abi.PREC_FP32)
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    md.forces(want=False)
+    ts, nsamp = md.surrogate_collect(24, 1, 1 << 20, 0)
+    assert nsamp == 24 * 4096 and ts == 24 * 8
+    md.mlp_init(7)
+    l1 = md.surrogate_train(600, batch=4096, lr=1e-3, seed=1)
+    l2 = md.surrogate_train(200, batch=4096, lr=2e-4, seed=2)
+    losses = np.concatenate([l1, l2])
+    say(f"training loss: first {losses[0]:.4e}, last {losses[-1]:.4e}")
+    yield md, ts, losses
+    md.close()
+
+
+def test_training_reduces_loss(gpu, trained):
+    md, ts, losses = trained
+    assert np.all(np.isfinite(losses))
+    assert losses[-20:].mean() < 0.2 * losses[:5].mean()
+
+
+def test_trained_network_beats_zero_drift(gpu, trained):
+    md, ts, _ = trained
+    ts, rms, mx = md.surrogate_eval(ts)
+    say(f"8-step displacement error vs plain (rms / max, sigma): network {rms[0]:.3e} / {mx[0]:.3e}, "
+        f"zero drift {rms[1]:.3e} / {mx[1]:.3e}, one Euler step of 8 dt {rms[2]:.3e} / {mx[2]:.3e}")
+    assert rms[0] < 0.5 * rms[1]
+
+
+def test_surrogate_step_judge_and_fallback(gpu, trained):
+    md, ts, _ = trained
+    md.surrogate_stats_reset()
+    ts1, st = md.surrogate_step(8, ts, judge_every=4, tol=1e9)
+    say(f"tolerant judge: {st}")
+    assert ts1 == ts + 64
+    assert st["judged"] == 2 and st["failed"] == 0 and st["macro_steps"] == 6 and st["plain_steps"] == 16
+    md.surrogate_stats_reset()
+    ts2, st = md.surrogate_step(6, ts1, judge_every=2, tol=0.0)
+    say(f"strict judge: {st}")
+    # m0 surrogate, m1 judged+failed, m2 m3 fallback, m4 judged+failed, m5 fallback
+    assert ts2 == ts1 + 48
+    assert st["judged"] == 2 and st["failed"] == 2 and st["macro_steps"] == 1
+    assert st["plain_steps"] == 8 * 5
+    # state stays usable by the plain integrator afterwards
+    ts3 = md.step(3, ts2)
+    assert ts3 == ts2 + 3
+
+
+def test_surrogate_step_deterministic(gpu, trained):
+    md, ts, _ = trained
+    params = md.mlp_get_params()
+    p, box, col = configs.relaxed_bulk_3d(2048, seed=9, precision=abi.PREC_FP32)
+    outs = []
+    for _ in range(2):
+        m2 = HipMD(p, box, [sp["capacity"] for sp in col.species])
+        m2.upload(col)
+        m2.forces(want=False)
+        m2.mlp_set_params(params)
+        m2.surrogate_step(5, 0, judge_every=0, tol=1.0)
+        c = col.copy()
+        m2.download(c)
+        outs.append(c.flat("pos").copy())
+        m2.close()
+    assert np.array_equal(outs[0], outs[1])
+    assert np.all(np.isfinite(outs[0]))
