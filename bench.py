@@ -20,7 +20,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from cil_cuda_amd import abi, configs  # noqa: E402
+from cil_cuda_amd import abi, configs, surrogate  # noqa: E402
 from cil_cuda_amd.colony import HipMD  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -75,21 +75,17 @@ MLP_FLOP_PER_PARTICLE = 2 * (72 * 128 + 128 * 128 + 128 * 3)  # 51,968 useful FL
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
-def surrogate_leg(md, ts, n, plain_value, args):
+def surrogate_leg(md, ts, n, plain_value, args, p, box, col):
     """Train the drift surrogate on the GPU from plain trajectories of the same state, then
     time the surrogate-assisted step (judge every 16th macro-step, fallback above tol)."""
+    floor, drift = surrogate.noise_floor(p, box, col)
+    log(f"surrogate: deterministic 8-step displacement rms {drift:.3e}; spread over noise (floor of any "
+        f"start-state predictor) {floor:.3e}")
     t0 = time.perf_counter()
-    stride = max(1, n // (1 << 17))
-    nmacro = max(1, (2 << 20) // ((n + stride - 1) // stride))
-    ts, nsamp = md.surrogate_collect(nmacro, stride, 2 << 20, ts)
-    log(f"surrogate: {nsamp} samples from {nmacro} plain macro-steps in {time.perf_counter() - t0:.1f} s")
-    md.mlp_init(1)
-    t0 = time.perf_counter()
-    k1 = int(args.train_steps * 0.7)
-    losses = np.concatenate([md.surrogate_train(k1, batch=8192, lr=1e-3, seed=1),
-                             md.surrogate_train(args.train_steps - k1, batch=8192, lr=2e-4, seed=2)])
+    ts, losses = surrogate.train(md, ts, 2 << 20, 16, args.train_steps, batch=8192, lr=1e-3, log=log)
     t_train = time.perf_counter() - t0
-    log(f"surrogate: {args.train_steps} Adam steps in {t_train:.1f} s, loss {losses[0]:.3e} -> "
+    nsamp = 2 << 20
+    log(f"surrogate: collection + {args.train_steps} Adam steps in {t_train:.1f} s, loss {losses[0]:.3e} -> "
         f"{losses[-50:].mean():.3e}")
     ts, rms, mx = md.surrogate_eval(ts)
     log(f"surrogate: 8-step error rms/max network {rms[0]:.3e}/{mx[0]:.3e} zero-drift {rms[1]:.3e}/{mx[1]:.3e} "
@@ -118,6 +114,7 @@ def surrogate_leg(md, ts, n, plain_value, args):
         "macro_steps": nmac, "judge_every": args.judge_every, "tol_sigma": args.tol, "stats": st,
         "eval_err_rms": {"network": rms[0], "zero_drift": rms[1], "euler_8dt": rms[2]},
         "eval_err_max": {"network": mx[0], "zero_drift": mx[1], "euler_8dt": mx[2]},
+        "noise_floor_rms": floor, "drift_rms": drift,
         "training": {"samples": nsamp, "adam_steps": args.train_steps, "batch": 8192, "seconds": t_train,
                      "loss_first": float(losses[0]), "loss_last50": float(losses[-50:].mean())},
         "kernels_us": {k: kus[k] for k in ("sort", "knn", "mlp", "macro") if k in kus},
@@ -191,7 +188,7 @@ def main():
         "step_hbm_gbs": STEP_BYTES_PER_PARTICLE * value / 1e9,
     }
     if not args.no_surrogate:
-        out["surrogate"] = surrogate_leg(md, ts, n, value, args)
+        out["surrogate"] = surrogate_leg(md, ts, n, value, args, p, box, col)
     md.close()
     if not args.no_cpu:
         try:

@@ -163,11 +163,19 @@ def lib():
     L.cil_hip_knn_features.argtypes = [vp, PF, C.c_size_t]
     L.cil_hip_surrogate_collect.argtypes = [vp, C.c_int, C.c_int, C.c_long, PI, PL]
     L.cil_hip_surrogate_set_samples.argtypes = [vp, PF, PF, C.c_long]
+    L.cil_hip_surrogate_get_samples.argtypes = [vp, PF, PF, C.c_long, PL]
     L.cil_hip_surrogate_clear_samples.argtypes = [vp]
     L.cil_hip_surrogate_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_uint64, PF]
     L.cil_hip_surrogate_eval.argtypes = [vp, PI, PD, PD]
     L.cil_hip_surrogate_step.argtypes = [vp, C.c_int, C.c_int, C.c_double, PI, C.POINTER(SurrogateStats)]
     L.cil_hip_surrogate_stats_reset.argtypes = [vp]
+    L.cil_hip_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
+    L.cil_hip_batch_destroy.argtypes = [vp]
+    L.cil_hip_batch_size.argtypes = [vp]
+    L.cil_hip_batch_step.argtypes = [vp, C.c_int, PI]
+    L.cil_hip_batch_autotune.argtypes = [vp, C.c_int, PI, PI, PI, PD]
+    L.cil_hip_batch_last_error.argtypes = [vp]
+    L.cil_hip_batch_last_error.restype = C.c_char_p
     L.cil_hip_last_error.argtypes = [vp]
     L.cil_hip_last_error.restype = C.c_char_p
     L.cil_hip_version.restype = C.c_char_p
@@ -179,7 +187,7 @@ def lib():
 
 def check_layout(L=None):
     L = L or lib()
-    want = {0: Params, 1: Species, 2: SpeciesView, 3: Box}
+    want = {0: Params, 1: Species, 2: SpeciesView, 3: Box, 4: SurrogateStats}
     for k, cls in want.items():
         got = L.cil_hip_sizeof(k)
         if got != C.sizeof(cls):
@@ -192,8 +200,11 @@ EXPORTS = (
     "cil_hip_forces", "cil_hip_pairs", "cil_hip_check_positions", "cil_hip_set_external_noise",
     "cil_hip_num_cells", "cil_hip_timings", "cil_hip_bin_info", "cil_hip_set_tuning", "cil_hip_set_timing", "cil_hip_set_kernel", "cil_hip_autotune",
     "cil_hip_sizeof", "cil_hip_last_error", "cil_hip_version",
+    "cil_hip_batch_create", "cil_hip_batch_destroy", "cil_hip_batch_size", "cil_hip_batch_step",
+    "cil_hip_batch_autotune", "cil_hip_batch_last_error",
     "cil_hip_mlp_nparams", "cil_hip_mlp_set_params", "cil_hip_mlp_get_params", "cil_hip_mlp_init",
     "cil_hip_mlp_forward", "cil_hip_mlp_grad", "cil_hip_knn_features", "cil_hip_surrogate_collect",
-    "cil_hip_surrogate_set_samples", "cil_hip_surrogate_clear_samples", "cil_hip_surrogate_train", "cil_hip_surrogate_eval",
+    "cil_hip_surrogate_set_samples", "cil_hip_surrogate_get_samples",
+    "cil_hip_surrogate_clear_samples", "cil_hip_surrogate_train", "cil_hip_surrogate_eval",
     "cil_hip_surrogate_step", "cil_hip_surrogate_stats_reset",
 )

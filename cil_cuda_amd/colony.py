@@ -262,6 +262,19 @@ class HipMD:
         _check(self.L, self.ctx, self.L.cil_hip_surrogate_set_samples(self.ctx, _fp(X), _fp(T), X.shape[0]),
                "surrogate_set_samples")
 
+    def surrogate_get_samples(self, with_x=True):
+        n = C.c_long()
+        _check(self.L, self.ctx, self.L.cil_hip_surrogate_get_samples(self.ctx, None, None, 0, C.byref(n)),
+               "surrogate_get_samples")
+        X = np.zeros((n.value if with_x else 0, abi.MLP_NFEAT), dtype=np.float32)
+        T = np.zeros((n.value, 3), dtype=np.float32)
+        _check(self.L, self.ctx, self.L.cil_hip_surrogate_get_samples(self.ctx, _fp(X) if with_x else None, _fp(T),
+                                                                      n.value, C.byref(n)), "surrogate_get_samples")
+        return (X if with_x else None), T
+
+    def surrogate_clear_samples(self):
+        _check(self.L, self.ctx, self.L.cil_hip_surrogate_clear_samples(self.ctx), "surrogate_clear_samples")
+
     def surrogate_train(self, nsteps, batch=4096, lr=1e-3, seed=0):
         losses = np.zeros(max(nsteps, 1), dtype=np.float32)
         _check(self.L, self.ctx, self.L.cil_hip_surrogate_train(self.ctx, int(nsteps), int(batch), float(lr), int(seed),
@@ -286,6 +299,48 @@ class HipMD:
 
     def surrogate_stats_reset(self):
         _check(self.L, self.ctx, self.L.cil_hip_surrogate_stats_reset(self.ctx), "surrogate_stats_reset")
+
+
+class HipBatch:
+    """Many HipMD colonies advanced together, one launch per kernel (cil_hip_batch_*).
+    The member contexts stay usable on their own (upload, download, check_positions)."""
+
+    def __init__(self, members):
+        self.L = abi.lib()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.ctx.value for m in self.members])
+        self.b = C.c_void_p()
+        rc = self.L.cil_hip_batch_create(arr, len(self.members), C.byref(self.b))
+        if rc != 0:
+            raise abi.CilError(rc, "cil_hip_batch_create: the colonies differ in a launch-shaping setting "
+                                   "(dimension, precision, division, noise kernel, LJ, pair kernel, block, rng)")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise abi.CilError(rc, f"{what}: {self.L.cil_hip_batch_last_error(self.b).decode()}")
+
+    def step(self, nsteps, ts):
+        t = (C.c_int * len(self.members))(*[int(x) for x in ts])
+        self._check(self.L.cil_hip_batch_step(self.b, int(nsteps), t), "batch_step")
+        return [t[k] for k in range(len(self.members))]
+
+    def autotune(self, reps=3):
+        b, k, v = C.c_int(), C.c_int(), C.c_int()
+        us = C.c_double()
+        self._check(self.L.cil_hip_batch_autotune(self.b, int(reps), C.byref(b), C.byref(k), C.byref(v), C.byref(us)),
+                    "batch_autotune")
+        return {"bins_per_cutoff": b.value, "block": k.value, "variant": v.value, "force_pass_us": us.value}
+
+    def close(self):
+        if self.b:
+            self.L.cil_hip_batch_destroy(self.b)
+            self.b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _fp(a):

@@ -34,9 +34,9 @@ using namespace cil;
 // ======================================================================================
 // Bin keys
 // ======================================================================================
-__global__ void k_keys(const StepParams* __restrict__ Pp, Dev S) {
+__device__ inline void k_keys_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
   const StepParams& P = *Pp;
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int p = bid * blockDim.x + threadIdx.x;
   if (p >= S.P) return;
   int s, c;
   int cslot = p / P.nE;
@@ -58,6 +58,7 @@ __global__ void k_keys(const StepParams* __restrict__ Pp, Dev S) {
   S.key[p] = b;
   S.rank[p] = atomicAdd(&S.count[b], 1);
 }
+__global__ void k_keys(const StepParams* __restrict__ Pp, Dev S) { k_keys_body(Pp, S, (int)blockIdx.x); }
 
 // ======================================================================================
 // Exclusive prefix sum (3 small kernels): 4096 items per block of 256 threads.
@@ -92,9 +93,9 @@ __device__ inline int block_exclusive_scan(int v, int* lds, int* total) {
   return res;
 }
 
-__global__ void k_scan_local(const int* __restrict__ in, int* __restrict__ out, int* __restrict__ bsum, int n) {
+__device__ inline void k_scan_local_body(const int* __restrict__ in, int* __restrict__ out, int* __restrict__ bsum, int n, int bid) {
   __shared__ int lds[WAVE + 1];
-  int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  int base = bid * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   int v[SCAN_ITEMS];
   int sum = 0;
 #pragma unroll
@@ -109,10 +110,11 @@ __global__ void k_scan_local(const int* __restrict__ in, int* __restrict__ out, 
     if (base + k < n) out[base + k] = ex;
     ex += v[k];
   }
-  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+  if (threadIdx.x == 0) bsum[bid] = total;
 }
+__global__ void k_scan_local(const int* __restrict__ in, int* __restrict__ out, int* __restrict__ bsum, int n) { k_scan_local_body(in, out, bsum, n, (int)blockIdx.x); }
 
-__global__ void k_scan_top(int* __restrict__ bsum, int nb, int* __restrict__ total_out) {
+__device__ inline void k_scan_top_body(int* __restrict__ bsum, int nb, int* __restrict__ total_out, int bid) {
   __shared__ int lds[WAVE + 1];
   int carry = 0;
   for (int base = 0; base < nb; base += blockDim.x) {
@@ -125,22 +127,25 @@ __global__ void k_scan_top(int* __restrict__ bsum, int nb, int* __restrict__ tot
   }
   if (threadIdx.x == 0) *total_out = carry;
 }
+__global__ void k_scan_top(int* __restrict__ bsum, int nb, int* __restrict__ total_out) { k_scan_top_body(bsum, nb, total_out, (int)blockIdx.x); }
 
-__global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ bsum, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void k_scan_add_body(int* __restrict__ out, const int* __restrict__ bsum, int n, int bid) {
+  int i = bid * blockDim.x + threadIdx.x;
   if (i < n) out[i] += bsum[i / SCAN_TILE];
 }
+__global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ bsum, int n) { k_scan_add_body(out, bsum, n, (int)blockIdx.x); }
 
 // ======================================================================================
 // Scatter and per-bin sort + gather
 // ======================================================================================
-__global__ void k_scatter(Dev S) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void k_scatter_body(const Dev& S, int bid) {
+  int p = bid * blockDim.x + threadIdx.x;
   if (p >= S.P) return;
   int b = S.key[p];
   if (b < 0) return;
   S.idx[S.start[b] + S.rank[p]] = p;
 }
+__global__ void k_scatter(Dev S) { k_scatter_body(S, (int)blockIdx.x); }
 
 // Orders this wave's LDS accesses across lanes (wave-scope fence + wave barrier).
 __device__ inline void wave_lds_sync() {
@@ -186,11 +191,11 @@ __device__ inline void write_sorted<double>(const StepParams& P, const Dev& S, i
 }
 
 template <typename T>
-__global__ void __launch_bounds__(WAVE* BINSORT_WAVES) k_binsort(const StepParams* __restrict__ Pp, Dev S) {
+__device__ inline void k_binsort_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
   const StepParams& P = *Pp;
   __shared__ int buf[BINSORT_WAVES][BINSORT_MAX];
   int wid = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
-  int u = blockIdx.x * BINSORT_WAVES + wid;  // bucket (type, spatial bin)
+  int u = bid * BINSORT_WAVES + wid;  // bucket (type, spatial bin)
   if (u >= P.nbuckets) return;
   int s0 = S.start[u], n = S.start[u + 1] - s0;
   if (lane == 0) S.count[u] = 0;  // reset for the next step's counting pass
@@ -264,6 +269,8 @@ __global__ void __launch_bounds__(WAVE* BINSORT_WAVES) k_binsort(const StepParam
     write_sorted<T>(P, S, b, s0 + i, p);
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(WAVE* BINSORT_WAVES) k_binsort(const StepParams* __restrict__ Pp, Dev S) { k_binsort_body<T>(Pp, S, (int)blockIdx.x); }
 
 // ======================================================================================
 // Pair forces.  Thread per sorted particle: a wavefront holds 64 consecutive particles of
@@ -505,11 +512,11 @@ __device__ inline int z_segments(const StepParams& P, int bz, int R, int* z0, in
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_pair(const StepParams* __restrict__ Pp, Dev S) {
+__device__ inline void k_pair_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
   const StepParams& P = *Pp;
   const int D = P.box.dim;
   const int total = S.start[P.nbuckets];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = bid * blockDim.x + threadIdx.x;
   if (k >= total) return;
   const uint32_t ts = (uint32_t)*S.ts;
   const int meta = S.smeta[k];
@@ -611,6 +618,8 @@ __global__ void __launch_bounds__(256) k_pair(const StepParams* __restrict__ Pp,
   for (int dd = 0; dd < D; ++dd) fp[(size_t)dd * S.P + k] = F[dd];
   ((T*)S.ppair)[k] = pr;
 }
+template <typename T>
+__global__ void __launch_bounds__(256) k_pair(const StepParams* __restrict__ Pp, Dev S) { k_pair_body<T>(Pp, S, (int)blockIdx.x); }
 
 // Lean FP32 pair kernel (fast mode), specialised on the dimension.  Waves are laid out per
 // element type (the sorted array is type-major), so the target type, its species, the law
@@ -619,13 +628,13 @@ __global__ void __launch_bounds__(256) k_pair(const StepParams* __restrict__ Pp,
 // Single-disc cells: the same-cell exclusion is j == k, so the inner loop reads only the
 // 16-byte sorted record of each candidate.
 template <int D>
-__global__ void __launch_bounds__(256) k_pair32(const StepParams* __restrict__ Pp, Dev S) {
+__device__ inline void k_pair32_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
   const StepParams& P = *Pp;
   const float4* __restrict__ spos = (const float4*)S.spos;
   const int* __restrict__ smeta = S.smeta;
   const int* __restrict__ start = S.start;
   const int lane = threadIdx.x & (WAVE - 1);
-  const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE);
+  const int w = __builtin_amdgcn_readfirstlane(bid * (blockDim.x / WAVE) + threadIdx.x / WAVE);
   // wave -> (type, chunk of that type's sorted range)
   int tp = -1, k0 = 0, k1 = 0, acc = 0;
   for (int t = 0; t < P.ntypes; ++t) {
@@ -789,14 +798,16 @@ __global__ void __launch_bounds__(256) k_pair32(const StepParams* __restrict__ P
   S.gcount[k] = nguard;
   if (nguard > GUARD_SLOTS) atomicOr(S.err, (unsigned)ERR_GUARD);
 }
+template <int D>
+__global__ void __launch_bounds__(256) k_pair32(const StepParams* __restrict__ Pp, Dev S) { k_pair32_body<D>(Pp, S, (int)blockIdx.x); }
 
 // LJ pairs whose FP32 distance fell within 1e-5 of the cut-off: exact FP64 decision
 // (LJParams::elemForce predicate r^2 <= RCUT^2 sigma_ij^2 on the canonical positions), then
 // the FP32 force added by the thread that owns the target particle (deterministic).
-__global__ void k_lj_guard_fix(const StepParams* __restrict__ Pp, Dev S) {
+__device__ inline void k_lj_guard_fix_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
   const StepParams& P = *Pp;
   const int total = S.start[P.nbuckets];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = bid * blockDim.x + threadIdx.x;
   if (k >= total) return;
   const int n = S.gcount[k];
   if (n == 0) return;
@@ -834,13 +845,14 @@ __global__ void k_lj_guard_fix(const StepParams* __restrict__ Pp, Dev S) {
   }
   for (int d = 0; d < D; ++d) fp[(size_t)d * S.P + k] += F[d];
 }
+__global__ void k_lj_guard_fix(const StepParams* __restrict__ Pp, Dev S) { k_lj_guard_fix_body(Pp, S, (int)blockIdx.x); }
 
 // ======================================================================================
 // Noise draws (updateWienerProcess, CIL.cu:793-818), reference noise modes only.
 // ======================================================================================
-__global__ void k_noise(const StepParams* __restrict__ Pp, Dev S) {
+__device__ inline void k_noise_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
   const StepParams& P = *Pp;
-  int cslot = blockIdx.x * blockDim.x + threadIdx.x;
+  int cslot = bid * blockDim.x + threadIdx.x;
   if (cslot >= S.C) return;
   int s, c;
   if (!cell_active(P, S.ncells, cslot, &s, &c)) return;
@@ -909,6 +921,7 @@ __global__ void k_noise(const StepParams* __restrict__ Pp, Dev S) {
     S.awiener[(size_t)cslot * 2 + 1] = wa;
   }
 }
+__global__ void k_noise(const StepParams* __restrict__ Pp, Dev S) { k_noise_body(Pp, S, (int)blockIdx.x); }
 
 // ======================================================================================
 // Forces of one cell at the last force evaluation: intra (lazy) + pair (sorted).
@@ -940,10 +953,9 @@ __device__ inline void cell_total_force(const StepParams& P, const Dev& S, int s
 // Fused update (updateVelocityPositions, CIL.cu:902-939), one thread per cell slot.
 // ======================================================================================
 template <typename T>
-__global__ void __launch_bounds__(256) k_update(const StepParams* __restrict__ Pp, Dev S, int materialized,
-                                                int emit_keys) {
+__device__ inline void k_update_body(const StepParams* __restrict__ Pp, const Dev& S, int materialized, int emit_keys, int bid) {
   const StepParams& P = *Pp;
-  int cslot = blockIdx.x * blockDim.x + threadIdx.x;
+  int cslot = bid * blockDim.x + threadIdx.x;
   if (cslot >= S.C) return;
   int s, c;
   if (!cell_active(P, S.ncells, cslot, &s, &c)) {
@@ -1014,6 +1026,9 @@ __global__ void __launch_bounds__(256) k_update(const StepParams* __restrict__ P
     }
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(256) k_update(const StepParams* __restrict__ Pp, Dev S, int materialized,
+                                                int emit_keys) { k_update_body<T>(Pp, S, materialized, emit_keys, (int)blockIdx.x); }
 
 // Forces at the current state into the canonical FP64 frc array (for download / parity).
 template <typename T>
@@ -1129,4 +1144,5 @@ __global__ void k_pairs_list(const StepParams* __restrict__ Pp, Dev S, int reach
 #include "cil_division.inc"
 #include "cil_surrogate.inc"
 #include "cil_context.inc"
+#include "cil_batch.inc"
 #include "cil_surrogate_host.inc"

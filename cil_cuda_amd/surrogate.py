@@ -1,0 +1,53 @@
+"""Drift-surrogate workflow over the C ABI: training from plain trajectories and the noise
+floor of any predictor that sees only the macro-step's start state.
+
+The surrogate replaces 8 Euler steps by x += D(x0) + (exact sum of the 8 noise increments).
+The plain method's deterministic displacement over those 8 steps, sum_k dt mu F(x_k), depends
+on the noise through the intermediate positions x_k, so no function of x0 alone can match it
+better than its spread over noise realisations.  `noise_floor` measures that spread by running
+the same start state with two noise seeds: E|D_a - D_b|^2 / 2 is the conditional variance, and
+its square root is the smallest RMS error any x0-only predictor can reach.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import abi
+from .colony import HipMD
+
+
+def train(md: HipMD, ts: int, n_samples: int, nmacro: int, steps: int, batch: int = 8192, lr: float = 1e-3,
+          init_seed: int = 1, log=None):
+    """Collect `n_samples` (>= 1 per particle row) from `nmacro` plain macro-steps and run
+    `steps` Adam steps (70 % at lr, 30 % at lr / 5).  Returns (ts, losses)."""
+    n = sum(md.num_cells()) * md.nelem
+    stride = max(1, (n * nmacro) // max(1, n_samples))
+    ts, got = md.surrogate_collect(nmacro, stride, n_samples, ts)
+    if log:
+        log(f"surrogate: {got} samples from {nmacro} plain macro-steps (stride {stride})")
+    md.mlp_init(init_seed)
+    k1 = int(steps * 0.7)
+    losses = np.concatenate([md.surrogate_train(k1, batch=batch, lr=lr, seed=init_seed + 1),
+                             md.surrogate_train(steps - k1, batch=batch, lr=lr / 5, seed=init_seed + 2)])
+    return ts, losses
+
+
+def noise_floor(params: abi.Params, box: abi.Box, colony, ts: int = 0, seeds=(101, 202)):
+    """RMS (per particle, length units) of the deterministic 8-step displacement's spread over
+    noise realisations from the same start state, and the RMS of that displacement itself."""
+    T = []
+    for seed in seeds:
+        q = abi.Params.from_buffer_copy(params)
+        q.seed = seed
+        md = HipMD(q, box, [sp["capacity"] for sp in colony.species])
+        try:
+            md.upload(colony)
+            md.forces(want=False)
+            md.surrogate_collect(1, 1, colony.total_cells(), ts)
+            T.append(md.surrogate_get_samples(with_x=False)[1].astype(np.float64))
+        finally:
+            md.close()
+    d = T[0] - T[1]
+    floor = float(np.sqrt(np.mean((d * d).sum(1)) / 2.0))
+    drift = float(np.sqrt(np.mean(0.5 * ((T[0] ** 2).sum(1) + (T[1] ** 2).sum(1)))))
+    return floor, drift

@@ -251,12 +251,39 @@ int cil_hip_autotune(cil_ctx* ctx, int reps, int* best_bins, int* best_block, in
 /* Per-kernel-family HIP-event timing (project addition; off by default). */
 int cil_hip_set_timing(cil_ctx* ctx, int on);
 
-/* sizeof the ABI structs (0: cil_params, 1: cil_species, 2: cil_species_view, 3: cil_box),
+/* sizeof the ABI structs (0: cil_params, 1: cil_species, 2: cil_species_view, 3: cil_box,
+ * 4: cil_surrogate_stats),
  * so an FFI binding can verify its layout (project addition). */
 size_t cil_hip_sizeof(int which);
 
 const char* cil_hip_last_error(cil_ctx* ctx);
 const char* cil_hip_version(void);
+
+/* ---------------------------------------------------------------------------------------
+ * Batches of independent colonies (project addition; the reference runs one colony per
+ * process, CIL.cu:1072-1120).
+ *
+ * A batch advances many contexts (one colony each: own parameters, box, seed, bins and
+ * buffers) with one launch per kernel for all of them and one HIP graph per step loop.
+ * Every colony evolves exactly as it would under cil_hip_step alone.  The contexts must share
+ * the device, dimension, force precision, division model, noise kernel (cell Wiener / active
+ * noise or not), LJ presence, pair kernel variant and block size, and use counter-based RNG.
+ * They stay owned by the caller and must outlive the batch; cil_hip_upload / download /
+ * check_positions on a member context remain valid between batch steps.
+ * ------------------------------------------------------------------------------------- */
+typedef struct cil_batch cil_batch;
+
+int cil_hip_batch_create(cil_ctx* const* colonies, int n, cil_batch** out);
+int cil_hip_batch_destroy(cil_batch* batch);
+int cil_hip_batch_size(cil_batch* batch);
+/* nsteps Euler steps of every colony; ts[c] is colony c's step counter (in/out).  Every
+ * colony needs initialised forces (cil_hip_forces). */
+int cil_hip_batch_step(cil_batch* batch, int nsteps, int* ts);
+/* Times the batch force pass for bins per cut-off 1..3 x pair block 64/128/256 (x FP32
+ * kernel variant) and applies the fastest to every colony. */
+int cil_hip_batch_autotune(cil_batch* batch, int reps, int* best_bins, int* best_block, int* best_variant,
+                           double* best_us);
+const char* cil_hip_batch_last_error(cil_batch* batch);
 
 /* ---------------------------------------------------------------------------------------
  * Drift surrogate (project addition; no counterpart in the reference).
@@ -303,6 +330,8 @@ int cil_hip_surrogate_collect(cil_ctx* ctx, int nmacro, int stride, long cap, in
 /* Replaces the training set with n host samples: X (n, 72) rounded to BF16, T (n, 3) drift
  * in length units. */
 int cil_hip_surrogate_set_samples(cil_ctx* ctx, const float* X, const float* T, long n);
+/* Copies up to cap training samples back (X as float, T in length units); *n = samples held. */
+int cil_hip_surrogate_get_samples(cil_ctx* ctx, float* X, float* T, long cap, long* n);
 int cil_hip_surrogate_clear_samples(cil_ctx* ctx);
 /* nsteps Adam steps (beta 0.9/0.999, eps 1e-8) on minibatches of `batch` samples drawn with
  * replacement (Philox keyed by seed); losses[it] receives each minibatch loss (may be NULL). */

@@ -2,9 +2,12 @@
 
 Restates what the HIP surrogate kernels compute, so they can be checked:
   - features: for each single-disc particle, its 16 nearest neighbours (minimum image,
-    ties broken by particle index) in ascending distance, each as (dx, dy, dz, s_j) with
-    s_j = +sigma_j for species 0 and -sigma_j for species 1; then sigma_i, species_i, the
-    cell axis (3 zeros for single discs) and 3 zero pads -> 72 features;
+    ties broken by particle index) in ascending distance, each as (u_x m, u_y m, u_z m, g)
+    with u = d / r the unit vector towards the neighbour, m = sigma_j for species 0 and
+    sigma_j + 4 for species 1, g = r / sigma_bar - 1 (sigma_bar = (sigma_i + sigma_j) / 2);
+    then sigma_i, species_i, the cell axis (3 zeros for single discs) and 3 zero pads -> 72
+    features.  (A bijective re-expression of (dx, dy, dz, sigma_j, species_j) that BF16
+    stores to ~0.4 % relative.)
   - MLP 72 -> 128 -> 128 -> 3, SiLU after the hidden layers, output scaled by OUT_SCALE;
   - loss: mean over samples and components of (y - t)^2 / 2 on the scaled target;
   - backward and an Adam step (Kingma & Ba 2015).
@@ -40,11 +43,22 @@ def knn_features(pos, sigma, species, box_len, k=NNB):
         jj = jj[order]
         dx = x[jj] - x[i]
         dx -= L * np.round(dx / L)
-        s = np.where(species[jj] == 0, sigma[jj], -sigma[jj])
-        feats[i, : 4 * k] = np.concatenate([dx, s[:, None]], axis=1).reshape(-1)
+        feats[i, : 4 * k] = encode_neighbours(dx, sigma[i], sigma[jj], species[jj]).reshape(-1)
         feats[i, 64] = sigma[i]
         feats[i, 65] = species[i]
     return feats
+
+
+SPECIES1_SHIFT = 4.0
+
+
+def encode_neighbours(dx, sigma_i, sigma_j, species_j):
+    """(k, 3) displacements -> (k, 4) rows (u m, g) as described in the module header."""
+    r = np.sqrt((dx * dx).sum(1))
+    m = np.where(species_j == 0, sigma_j, sigma_j + SPECIES1_SHIFT)
+    sc = np.where(r > 0, m / np.where(r > 0, r, 1.0), 0.0)
+    g = r / (0.5 * (sigma_i + sigma_j)) - 1.0
+    return np.concatenate([dx * sc[:, None], g[:, None]], axis=1)
 
 
 def silu(z):

@@ -46,8 +46,16 @@ def test_knn_features_against_brute_force():
         r = np.sqrt((d * d).sum(1))
         r[i] = np.inf
         order = np.lexsort((np.arange(n), r))[:16]
-        expect = np.concatenate([d[order], np.where(sp[order] == 0, sig[order], -sig[order])[:, None]], 1)
-        assert np.allclose(F[i, :64].reshape(16, 4), expect)
+        # decode the rows back to (d, sigma_j, species_j) and compare with the brute force
+        rows = F[i, :64].reshape(16, 4)
+        m = np.sqrt((rows[:, :3] ** 2).sum(1))
+        spj = (m > 3.0).astype(int)
+        sgj = m - 4.0 * spj
+        rj = (rows[:, 3] + 1.0) * 0.5 * (sig[i] + sgj)
+        dj = rows[:, :3] / m[:, None] * rj[:, None]
+        assert np.array_equal(spj, sp[order])
+        assert np.allclose(sgj, sig[order])
+        assert np.allclose(dj, d[order])
         assert F[i, 64] == sig[i] and F[i, 65] == sp[i]
         assert np.all(F[i, 66:] == 0)
 
