@@ -90,27 +90,32 @@ def surrogate_leg(md, ts, n, plain_value, args, p, box, col):
     ts, rms, mx = md.surrogate_eval(ts)
     log(f"surrogate: 8-step error rms/max network {rms[0]:.3e}/{mx[0]:.3e} zero-drift {rms[1]:.3e}/{mx[1]:.3e} "
         f"euler-8dt {rms[2]:.3e}/{mx[2]:.3e}")
-    # per-kernel averages (timing mode synchronises after every macro-step)
-    md.set_timing(True)
-    ts, _ = md.surrogate_step(8, ts, judge_every=0, tol=args.tol)
-    kus = md.timings()
-    md.set_timing(False)
-    md.surrogate_stats_reset()
     nmac = max(args.judge_every, (args.steps + 7) // 8)
-    ts, _ = md.surrogate_step(2, ts, judge_every=0, tol=args.tol)  # warm-up
-    md.num_cells()
-    md.surrogate_stats_reset()
-    t0 = time.perf_counter()
-    ts, st = md.surrogate_step(nmac, ts, judge_every=args.judge_every, tol=args.tol)
-    md.num_cells()
-    el = time.perf_counter() - t0
-    value = n * 8 * nmac / el
-    log(f"surrogate: {nmac} macro-steps in {el * 1e3:.1f} ms -> {value:.3e} particle-steps/s "
-        f"({value / plain_value:.2f}x plain); stats {st}")
+    err = None
+    kus, st, value = {}, {}, None
+    try:
+        # per-kernel averages (timing mode synchronises after every macro-step)
+        md.set_timing(True)
+        ts, _ = md.surrogate_step(args.judge_every, ts, judge_every=args.judge_every, tol=args.tol)
+        kus = md.timings()
+        md.set_timing(False)
+        md.surrogate_stats_reset()
+        md.num_cells()
+        t0 = time.perf_counter()
+        ts, st = md.surrogate_step(nmac, ts, judge_every=args.judge_every, tol=args.tol)
+        md.num_cells()
+        el = time.perf_counter() - t0
+        value = n * 8 * nmac / el
+        log(f"surrogate: {nmac} macro-steps in {el * 1e3:.1f} ms -> {value:.3e} particle-steps/s "
+            f"({value / plain_value:.2f}x plain); stats {st}")
+    except abi.CilError as e:
+        err = str(e)
+        log(f"surrogate: run failed: {err}")
     mlp_us = kus.get("mlp", 0.0)
     mfma_tf = MLP_FLOP_PER_PARTICLE * n / (mlp_us * 1e-6) / 1e12 if mlp_us > 0 else 0.0
     return {
-        "value": value, "unit": "particle-steps/s", "speedup_vs_plain": value / plain_value,
+        "value": value, "unit": "particle-steps/s", "speedup_vs_plain": value / plain_value if value else None,
+        "error": err,
         "macro_steps": nmac, "judge_every": args.judge_every, "tol_sigma": args.tol, "stats": st,
         "eval_err_rms": {"network": rms[0], "zero_drift": rms[1], "euler_8dt": rms[2]},
         "eval_err_max": {"network": mx[0], "zero_drift": mx[1], "euler_8dt": mx[2]},
@@ -123,6 +128,23 @@ def surrogate_leg(md, ts, n, plain_value, args, p, box, col):
     }
 
 
+def copy_calibration(nbytes=256 << 20, reps=3):
+    """Device-to-device copies of a known size: the FETCH_SIZE / WRITE_SIZE counters of the
+    runtime's copy kernel calibrate the counters of the library's kernels."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,10 +154,15 @@ def main():
     ap.add_argument("--bins-per-cutoff", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
-    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--judge-every", type=int, default=16)
     ap.add_argument("--tol", type=float, default=1e-3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 256 MiB device-to-device copies (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the multi-GPU "
               "sharding is a design (see DESIGN.md), not a run", file=sys.stderr)
@@ -161,6 +188,13 @@ def main():
     fam = md.timings()
     nb, w = md.bin_info()
     md.set_timing(False)
+    # the same number of steps through the graph-captured path (no events), for reference
+    md.num_cells()
+    t0 = time.perf_counter()
+    ts = md.step(args.steps, ts)
+    md.num_cells()
+    graph_elapsed = time.perf_counter() - t0
+    log(f"graph path: {graph_elapsed * 1e3 / args.steps:.3f} ms/step")
     n = args.n
     value = n * args.steps / elapsed
     pair_us = fam.get("pair", 0.0)
@@ -183,13 +217,27 @@ def main():
                    "particles": n, "bins": nb, "bin_width": w},
         "kernels_us": fam,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_pair<float>",
-                     "bytes_per_particle": PAIR_BYTES_PER_PARTICLE},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_pair32<3>",
+                     "kernel_us": pair_us, "bytes_per_particle": PAIR_BYTES_PER_PARTICLE},
         "step_hbm_gbs": STEP_BYTES_PER_PARTICLE * value / 1e9,
+        "timed_region": "steps launched directly with per-kernel HIP events (64-step chunks)",
+        "graph_path": {"ms_per_step": graph_elapsed * 1e3 / args.steps, "value": n * args.steps / graph_elapsed},
     }
     if not args.no_surrogate:
         out["surrogate"] = surrogate_leg(md, ts, n, value, args, p, box, col)
     md.close()
+    if not args.no_surrogate:
+        # the same relaxed state at kT = 0.005, where the start-state noise floor is below tol
+        q = abi.Params.from_buffer_copy(p)
+        for s_ in range(q.nspecies):
+            q.species[s_].kT = 0.005
+        m2 = HipMD(q, box, [sp["capacity"] for sp in col.species])
+        m2.upload(col)
+        m2.forces(want=False)
+        res = surrogate_leg(m2, 0, n, value, args, q, box, col)
+        res["kT"] = 0.005
+        out["surrogate_low_kT"] = res
+        m2.close()
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline()

@@ -290,9 +290,11 @@ const char* cil_hip_batch_last_error(cil_batch* batch);
  *
  * A per-particle MLP (72 -> 128 -> 128 -> 3, SiLU, BF16 weights and activations on MFMA,
  * FP32 accumulation) predicts the deterministic displacement over 8 Euler steps from the
- * 16 nearest neighbours: 16 x (dx, dy, dz, +-sigma_j) (+ for species 0), own sigma, own
- * species, 6 zeros.  A macro-step adds that drift plus the exact sum of the 8 steps' FDT
- * noise increments.  Supported: 3D periodic single-disc colonies in FP32 force mode, no
+ * 16 nearest neighbours, each given as (u m, 20 g): u the unit vector towards it, m = sigma_j
+ * (species 0) or sigma_j + 4 (species 1), g = r / sigma_bar - 1; then own sigma, own species
+ * and 6 zeros (72 features).  A macro-step adds that drift (capped at 0.05 sigma per particle,
+ * a trust bound) plus the exact sum of the 8 steps' FDT noise increments.  Supported: 3D
+ * periodic single-disc colonies in FP32 force mode, no
  * division, walls, motility, pressure friction, drag or active noise; other set-ups return
  * CIL_E_INVALID.  The kNN search visits shells of bins around each particle until the 16th
  * neighbour is covered, up to 3 shells (fewer in boxes of < 7 bins per dimension); a 16th
@@ -313,7 +315,8 @@ typedef struct cil_surrogate_stats {
 int cil_hip_mlp_nparams(void);
 int cil_hip_mlp_set_params(cil_ctx* ctx, const float* params, size_t n);
 int cil_hip_mlp_get_params(cil_ctx* ctx, float* params, size_t n);
-/* He-normal weights, zero biases, from Philox keyed by seed; resets the Adam state. */
+/* Weights and biases uniform in (-1/sqrt(fan_in), 1/sqrt(fan_in)), from Philox keyed by seed;
+ * resets the Adam state. */
 int cil_hip_mlp_init(cil_ctx* ctx, uint64_t seed);
 /* Y (n, 3) = drift in length units for features X (n, 72), rounded to BF16 on entry. */
 int cil_hip_mlp_forward(cil_ctx* ctx, const float* X, size_t n, float* Y);

@@ -8,6 +8,7 @@
 //   cil::hip::CellColony     per-species arrays in the reference layout, addCell/removeCell
 //   cil::hip::Session        owns one device context (upload, step, download)
 //   cil::hip::EulerHIP       `void (*)(int& ts)`-compatible step on the bound session
+//   cil::hip::Batch          many sessions stepped by one launch per kernel
 //
 // Errors surface as std::runtime_error, the reference's run_assert convention
 // (Utils.hpp:99-109).  Host arrays are never cached by the device side.
@@ -148,6 +149,31 @@ class Session {
  private:
   CellColony& col_;
   cil_ctx* ctx_ = nullptr;
+};
+
+// Many sessions advanced together, one launch per kernel for all colonies
+// (cil_hip_batch_*).  The sessions must outlive the batch.
+class Batch {
+ public:
+  explicit Batch(const std::vector<Session*>& sessions) {
+    std::vector<cil_ctx*> c;
+    for (Session* s : sessions) c.push_back(s->ctx());
+    run_assert(cil_hip_batch_create(c.data(), (int)c.size(), &b_), nullptr, "cil_hip_batch_create");
+  }
+  ~Batch() { cil_hip_batch_destroy(b_); }
+  Batch(const Batch&) = delete;
+  Batch& operator=(const Batch&) = delete;
+  // `gts` Euler steps of every colony; ts[c] is colony c's step counter
+  void steps(int gts, std::vector<int>& ts) {
+    int rc = cil_hip_batch_step(b_, gts, ts.data());
+    if (rc != CIL_OK)
+      throw std::runtime_error(std::string("cil_hip_batch_step: ") + cil_hip_batch_last_error(b_) + " (status " +
+                               std::to_string(rc) + ")");
+  }
+  cil_batch* handle() { return b_; }
+
+ private:
+  cil_batch* b_ = nullptr;
 };
 
 // The MDIntegrator-shaped entry point: bind a session once, then assign

@@ -2,7 +2,7 @@
 
 Restates what the HIP surrogate kernels compute, so they can be checked:
   - features: for each single-disc particle, its 16 nearest neighbours (minimum image,
-    ties broken by particle index) in ascending distance, each as (u_x m, u_y m, u_z m, g)
+    ties broken by particle index) in ascending distance, each as (u_x m, u_y m, u_z m, 20 g)
     with u = d / r the unit vector towards the neighbour, m = sigma_j for species 0 and
     sigma_j + 4 for species 1, g = r / sigma_bar - 1 (sigma_bar = (sigma_i + sigma_j) / 2);
     then sigma_i, species_i, the cell axis (3 zeros for single discs) and 3 zero pads -> 72
@@ -50,6 +50,7 @@ def knn_features(pos, sigma, species, box_len, k=NNB):
 
 
 SPECIES1_SHIFT = 4.0
+GAP_SCALE = 20.0
 
 
 def encode_neighbours(dx, sigma_i, sigma_j, species_j):
@@ -57,7 +58,7 @@ def encode_neighbours(dx, sigma_i, sigma_j, species_j):
     r = np.sqrt((dx * dx).sum(1))
     m = np.where(species_j == 0, sigma_j, sigma_j + SPECIES1_SHIFT)
     sc = np.where(r > 0, m / np.where(r > 0, r, 1.0), 0.0)
-    g = r / (0.5 * (sigma_i + sigma_j)) - 1.0
+    g = GAP_SCALE * (r / (0.5 * (sigma_i + sigma_j)) - 1.0)
     return np.concatenate([dx * sc[:, None], g[:, None]], axis=1)
 
 

@@ -47,7 +47,7 @@ int main() {
   box.length[0] = box.length[1] = L;
   box.periodic[0] = box.periodic[1] = 1;
 
-  cil::hip::CellColony gpu(2, 2, {n}), cpu(2, 2, {n});
+  cil::hip::CellColony gpu(2, 2, {n}), cpu(2, 2, {n}), b1(2, 2, {n}), b2(2, 2, {n});
   std::mt19937_64 rng(5);
   std::uniform_real_distribution<double> U(0.0, 1.0);
   for (int c = 0; c < n; ++c) {
@@ -65,6 +65,8 @@ int main() {
     gpu.species(0).rest_len[a] = cpu.species(0).rest_len[a] = 0.5;
     gpu.species(0).state[a] = cpu.species(0).state[a] = 1;
   }
+  b1 = gpu;  // batch members: the same colony (seed 7) and a second one with seed 8
+  b2 = gpu;
   try {
     cil::hip::Session session(p, box, gpu);
     session.upload();
@@ -93,7 +95,31 @@ int main() {
           err = std::fmax(err, std::fabs(x));
         }
     std::printf("ts=%d max position difference %.3e\n", ts, err);
-    return (ts == nsteps && err <= 1e-9) ? 0 : 1;
+    // the same colony inside a two-colony batch must match the single session bit for bit
+    cil_params p2 = p;
+    p2.seed = 8;
+    cil::hip::Session s1(p, box, b1), s2(p2, box, b2);
+    s1.upload();
+    s2.upload();
+    s1.calculForcesPressure();
+    s2.calculForcesPressure();
+    std::vector<int> tsb = {0, 0};
+    {
+      cil::hip::Batch batch({&s1, &s2});
+      batch.steps(nsteps, tsb);
+    }
+    s1.download();
+    s2.download();
+    bool same = true, differs = false;
+    for (int c = 0; c < n; ++c)
+      for (int e = 0; e < 2; ++e)
+        for (int d = 0; d < 2; ++d) {
+          same = same && b1.pos(0, c, e, d) == gpu.pos(0, c, e, d);
+          differs = differs || b2.pos(0, c, e, d) != gpu.pos(0, c, e, d);
+        }
+    std::printf("batch: colony 0 %s the single session, colony 1 (other seed) %s\n",
+                same ? "equals" : "DIFFERS FROM", differs ? "differs" : "DOES NOT DIFFER");
+    return (ts == nsteps && err <= 1e-9 && same && differs && tsb[0] == nsteps && tsb[1] == nsteps) ? 0 : 1;
   } catch (const std::runtime_error& e) {
     if (std::strstr(e.what(), "status -2")) {
       std::printf("no GPU device: %s\n", e.what());

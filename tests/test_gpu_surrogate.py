@@ -10,9 +10,9 @@ weights and inputs):
     >= 99.9 % of them;
   - kNN features: equal to the oracle's rows rounded to BF16 (one BF16 ulp) for every
     particle without a near-tie (relative gap < 1e-4) among its 17 nearest distances;
-  - training on plain trajectories: the trained network's 8-step displacement error is
-    below the zero-drift error (what the network gains over the method without it), and
-    the numbers are printed next to one Euler step of 8 dt.
+  - training on plain trajectories (kT = 0.005): the trained network's 8-step displacement
+    error is at least 10 % below the zero-drift error (what the network gains over the
+    method without it); printed next to one Euler step of 8 dt.
 """
 import sys
 
@@ -182,7 +182,7 @@ def test_knn_features_match_oracle(gpu):
             continue
         checked += 1
         want = bf16(ref[i])
-        tol = 2.0 ** -7 * np.abs(want) + 1e-6
+        tol = 2.0 ** -7 * np.abs(want) + 1e-5
         if not np.all(np.abs(F[i] - want) <= tol):
             bad = np.nonzero(np.abs(F[i] - want) > tol)[0]
             raise AssertionError(f"row {i}: features {bad[:8]} gpu {F[i][bad[:8]]} oracle {want[bad[:8]]}")
@@ -202,16 +202,18 @@ def test_surrogate_requires_supported_setup(gpu):
 
 @pytest.fixture(scope="module")
 def trained():
-    """A 4096-particle relaxed bulk, 24 plain macro-steps of training data, 800 Adam steps."""
-    p, box, col = configs.relaxed_bulk_3d(4096, seed=3, precision=abi.PREC_FP32)
+    """A 4096-particle relaxed bulk at kT = 0.005 (where the noise floor of start-state
+    predictors, ~3e-4 sigma, is well below the drift), 24 plain macro-steps of training data,
+    4000 Adam steps."""
+    p, box, col = configs.relaxed_bulk_3d(4096, seed=3, precision=abi.PREC_FP32, kT=0.005)
     md = HipMD(p, box, [sp["capacity"] for sp in col.species])
     md.upload(col)
     md.forces(want=False)
     ts, nsamp = md.surrogate_collect(24, 1, 1 << 20, 0)
     assert nsamp == 24 * 4096 and ts == 24 * 8
     md.mlp_init(7)
-    l1 = md.surrogate_train(600, batch=4096, lr=1e-3, seed=1)
-    l2 = md.surrogate_train(200, batch=4096, lr=2e-4, seed=2)
+    l1 = md.surrogate_train(3000, batch=4096, lr=1e-3, seed=1)
+    l2 = md.surrogate_train(1000, batch=4096, lr=2e-4, seed=2)
     losses = np.concatenate([l1, l2])
     say(f"training loss: first {losses[0]:.4e}, last {losses[-1]:.4e}")
     yield md, ts, losses
@@ -229,7 +231,7 @@ def test_trained_network_beats_zero_drift(gpu, trained):
     ts, rms, mx = md.surrogate_eval(ts)
     say(f"8-step displacement error vs plain (rms / max, sigma): network {rms[0]:.3e} / {mx[0]:.3e}, "
         f"zero drift {rms[1]:.3e} / {mx[1]:.3e}, one Euler step of 8 dt {rms[2]:.3e} / {mx[2]:.3e}")
-    assert rms[0] < rms[1]
+    assert rms[0] < 0.9 * rms[1]
 
 
 def test_noise_floor_of_start_state_predictors(gpu):

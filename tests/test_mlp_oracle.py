@@ -51,7 +51,7 @@ def test_knn_features_against_brute_force():
         m = np.sqrt((rows[:, :3] ** 2).sum(1))
         spj = (m > 3.0).astype(int)
         sgj = m - 4.0 * spj
-        rj = (rows[:, 3] + 1.0) * 0.5 * (sig[i] + sgj)
+        rj = (rows[:, 3] / M.GAP_SCALE + 1.0) * 0.5 * (sig[i] + sgj)
         dj = rows[:, :3] / m[:, None] * rj[:, None]
         assert np.array_equal(spj, sp[order])
         assert np.allclose(sgj, sig[order])
