@@ -54,7 +54,7 @@ def cpu_baseline(seed=1, n_sample=1 << 17):
     q.precision = abi.PREC_FP64
     threads = max(1, min(16, os.cpu_count() or 1))
     res = {}
-    for nth, nsteps in ((threads, 3), (1, 1)):
+    for nth, nsteps in ((threads, 30), (1, 4)):
         orc = O.Oracle(q, box, fast=True, traversal=0, threads=nth)
         c = col.copy()
         orc.forces(c, 0)
@@ -176,6 +176,10 @@ def main():
         md.set_tuning(args.bins_per_cutoff, 0)
     md.upload(col)
     md.forces(want=False)
+    tuned = None
+    if not args.bins_per_cutoff:
+        tuned = md.autotune(reps=3)
+        log(f"autotune: {tuned}")
     ts = md.step(args.warmup, 0)
     md.set_timing(True)
     md.num_cells()  # synchronise
@@ -214,7 +218,7 @@ def main():
         "data": "synthetic (seeded lognormal bulk, 100 push-off relaxation steps)",
         "config": {"workload": "3D periodic bulk, 1048576 single discs, phi=0.6, lognormal sigma CV 0.1, "
                                "LJ 2.5 sigma (species 0 pairs) + soft core, dt=1e-4, kT=0.05",
-                   "particles": n, "bins": nb, "bin_width": w},
+                   "particles": n, "bins": nb, "bin_width": w, "autotuned": tuned},
         "kernels_us": fam,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_pair32<3>",

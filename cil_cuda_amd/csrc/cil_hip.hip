@@ -31,6 +31,27 @@ using namespace cil;
 #define WAVE 64
 #define GUARD_SLOTS 4
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8) and each XCD
+// has its own L2.  Kernels whose consecutive workgroups work on neighbouring bins renumber
+// them so that each XCD processes one contiguous range: a bin's neighbour data is then
+// fetched into one L2 instead of eight.  A bijection of [0, nblk).
+#define NUM_XCD 8
+__device__ inline int xcd_block(int b, int nblk) {
+  const int q = nblk / NUM_XCD, r = nblk % NUM_XCD, x = b % NUM_XCD;
+  return x * q + (x < r ? x : r) + b / NUM_XCD;
+}
+// The same idea where work per workgroup is uneven along the index (type-major buckets put
+// all of one element type first): the grid is cut into 8*XCD_CHUNKS contiguous chunks dealt
+// round-robin to the XCDs, so every XCD gets a share of each region while a chunk's
+// workgroups still share one L2.  The tail beyond whole chunks keeps its own numbering.
+#define XCD_CHUNKS 16
+__device__ inline int xcd_block_chunked(int b, int nblk) {
+  const int C = nblk / (NUM_XCD * XCD_CHUNKS);
+  if (C == 0 || b >= C * NUM_XCD * XCD_CHUNKS) return b;
+  const int t = b / NUM_XCD, x = b % NUM_XCD;
+  return ((t / C) * NUM_XCD + x) * C + t % C;
+}
+
 // ======================================================================================
 // Bin keys
 // ======================================================================================
@@ -270,7 +291,7 @@ __device__ inline void k_binsort_body(const StepParams* __restrict__ Pp, const D
   }
 }
 template <typename T>
-__global__ void __launch_bounds__(WAVE* BINSORT_WAVES) k_binsort(const StepParams* __restrict__ Pp, Dev S) { k_binsort_body<T>(Pp, S, (int)blockIdx.x); }
+__global__ void __launch_bounds__(WAVE* BINSORT_WAVES) k_binsort(const StepParams* __restrict__ Pp, Dev S) { k_binsort_body<T>(Pp, S, xcd_block(blockIdx.x, gridDim.x)); }
 
 // ======================================================================================
 // Pair forces.  Thread per sorted particle: a wavefront holds 64 consecutive particles of
@@ -478,6 +499,46 @@ __device__ inline int bucket_of(const StepParams& P, int t, int x, int y, int z)
   return t * P.nbins + b;
 }
 
+// Up to two z-segments of the bins bz+olo .. bz+ohi (olo <= ohi, span <= ns in z), split at
+// the periodic seam; 0 segments when a non-periodic box clips the range away.
+__device__ inline int z_segments_range(const StepParams& P, int bz, int olo, int ohi, int* z0, int* z1, int* off) {
+  if (P.box.dim != 3) {
+    z0[0] = 0;
+    z1[0] = 0;
+    off[0] = 0;
+    return 1;
+  }
+  const int nz = P.ns[2];
+  int lo = bz + olo, hi = bz + ohi;
+  if (!P.box.pbc[2]) {
+    lo = lo < 0 ? 0 : lo;
+    hi = hi > nz - 1 ? nz - 1 : hi;
+    if (lo > hi) return 0;
+    z0[0] = lo; z1[0] = hi; off[0] = lo - bz;
+    return 1;
+  }
+  if (hi < 0) {
+    z0[0] = lo + nz; z1[0] = hi + nz; off[0] = lo - bz;
+    return 1;
+  }
+  if (lo > nz - 1) {
+    z0[0] = lo - nz; z1[0] = hi - nz; off[0] = lo - bz;
+    return 1;
+  }
+  if (lo < 0) {
+    z0[0] = lo + nz; z1[0] = nz - 1; off[0] = lo - bz;
+    z0[1] = 0; z1[1] = hi; off[1] = -bz;
+    return 2;
+  }
+  if (hi > nz - 1) {
+    z0[0] = lo; z1[0] = nz - 1; off[0] = lo - bz;
+    z0[1] = 0; z1[1] = hi - nz; off[1] = nz - bz;
+    return 2;
+  }
+  z0[0] = lo; z1[0] = hi; off[0] = lo - bz;
+  return 1;
+}
+
 // Up to two z-segments of the stencil column [bz-R, bz+R] (split at the periodic seam).
 // Segment k covers bins z0[k]..z1[k] whose unwrapped offset from bz starts at off[k].
 __device__ inline int z_segments(const StepParams& P, int bz, int R, int* z0, int* z1, int* off) {
@@ -619,7 +680,7 @@ __device__ inline void k_pair_body(const StepParams* __restrict__ Pp, const Dev&
   ((T*)S.ppair)[k] = pr;
 }
 template <typename T>
-__global__ void __launch_bounds__(256) k_pair(const StepParams* __restrict__ Pp, Dev S) { k_pair_body<T>(Pp, S, (int)blockIdx.x); }
+__global__ void __launch_bounds__(256) k_pair(const StepParams* __restrict__ Pp, Dev S) { k_pair_body<T>(Pp, S, xcd_block_chunked(blockIdx.x, gridDim.x)); }
 
 // Lean FP32 pair kernel (fast mode), specialised on the dimension.  Waves are laid out per
 // element type (the sorted array is type-major), so the target type, its species, the law
@@ -681,18 +742,37 @@ __device__ inline void k_pair32_body(const StepParams* __restrict__ Pp, const De
     const float cadh = (tq / nE == spec_p) ? 1.0f : ca_p;  // scale for f0 >= 0 (MD.cu:345-360)
     const bool same_species_cells = nE == 2 && (tq / nE == spec_p);
     const bool self_bucket = tq == tp;
+    // This particle's largest cut-off against type tq (diameter bound of tq), with a margin
+    // far above FP32 rounding: bins beyond it cannot hold a partner.  Columns are culled by
+    // their x/y gap and each z-run is clipped to the bins its remaining radius reaches.
+    const float rcp = fmaf(rca, me.w + (float)P.sigma_bound[tq], rcb) * 1.001f + 1e-6f;
+    const float rcp2 = rcp * rcp;
     int z0[2], z1[2], zo[2];
-    const int nseg = z_segments(P, bc[2], Rz, z0, z1, zo);
     for (int ox = -Rx; ox <= Rx; ++ox) {
       int nx = bc[0] + ox;
       if (P.box.pbc[0]) nx = nx < 0 ? nx + P.ns[0] : (nx >= P.ns[0] ? nx - P.ns[0] : nx);
       else if (nx < 0 || nx >= P.ns[0]) continue;
       const float qx = me.x - (float)ox * wx;
+      const float gx = mix ? 0.0f : (ox > 0 ? (float)ox * wx - me.x : (ox < 0 ? me.x - (float)(ox + 1) * wx : 0.0f));
+      const float gx2 = gx > 0.0f ? gx * gx : 0.0f;
+      if (gx2 > rcp2) continue;
       for (int oy = -Ry; oy <= Ry; ++oy) {
         int ny = bc[1] + oy;
         if (P.box.pbc[1]) ny = ny < 0 ? ny + P.ns[1] : (ny >= P.ns[1] ? ny - P.ns[1] : ny);
         else if (ny < 0 || ny >= P.ns[1]) continue;
         const float qy = me.y - (float)oy * wy;
+        const float gy = miy ? 0.0f : (oy > 0 ? (float)oy * wy - me.y : (oy < 0 ? me.y - (float)(oy + 1) * wy : 0.0f));
+        const float rem = rcp2 - gx2 - (gy > 0.0f ? gy * gy : 0.0f);
+        if (rem < 0.0f) continue;
+        int zlo = -Rz, zhi = Rz;
+        if (D == 3 && !miz) {
+          const float sr = sqrtf(rem);
+          const int a = (int)floorf((me.z + sr) / wz), b = (int)ceilf((me.z - sr) / wz) - 1;
+          zhi = a < Rz ? a : Rz;
+          zlo = b > -Rz ? b : -Rz;
+          if (zlo > zhi) continue;
+        }
+        const int nseg = z_segments_range(P, bc[2], zlo, zhi, z0, z1, zo);
         for (int sg = 0; sg < nseg; ++sg) {
           const int u0 = bucket_of(P, tq, nx, ny, z0[sg]);
           const int u1 = bucket_of(P, tq, nx, ny, z1[sg]);
@@ -799,7 +879,7 @@ __device__ inline void k_pair32_body(const StepParams* __restrict__ Pp, const De
   if (nguard > GUARD_SLOTS) atomicOr(S.err, (unsigned)ERR_GUARD);
 }
 template <int D>
-__global__ void __launch_bounds__(256) k_pair32(const StepParams* __restrict__ Pp, Dev S) { k_pair32_body<D>(Pp, S, (int)blockIdx.x); }
+__global__ void __launch_bounds__(256) k_pair32(const StepParams* __restrict__ Pp, Dev S) { k_pair32_body<D>(Pp, S, xcd_block_chunked(blockIdx.x, gridDim.x)); }
 
 // LJ pairs whose FP32 distance fell within 1e-5 of the cut-off: exact FP64 decision
 // (LJParams::elemForce predicate r^2 <= RCUT^2 sigma_ij^2 on the canonical positions), then
