@@ -151,7 +151,9 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=N_PARTICLES)
-    ap.add_argument("--bins-per-cutoff", type=int, default=0)
+    ap.add_argument("--bins-per-cutoff", type=int, default=0,
+                    help="pin the bins per cut-off (skips the autotuner)")
+    ap.add_argument("--block", type=int, default=0, help="pin the pair kernel block size (with --bins-per-cutoff)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--train-steps", type=int, default=20000)
@@ -173,7 +175,7 @@ def main():
     log(f"relaxed {args.n} particles in {time.perf_counter() - t0:.1f} s")
     md = HipMD(p, box, [sp["capacity"] for sp in col.species])
     if args.bins_per_cutoff:
-        md.set_tuning(args.bins_per_cutoff, 0)
+        md.set_tuning(args.bins_per_cutoff, args.block)
     md.upload(col)
     md.forces(want=False)
     tuned = None
@@ -203,6 +205,10 @@ def main():
     value = n * args.steps / elapsed
     pair_us = fam.get("pair", 0.0)
     achieved = PAIR_BYTES_PER_PARTICLE * n / (pair_us * 1e-6) / 1e9 if pair_us > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pair32_1M_hbm_traffic.json")
+    if n == N_PARTICLES and os.path.exists(tpath):  # FETCH/WRITE counter runs, copy-calibrated
+        traffic = json.load(open(tpath))["traffic_bytes_per_launch"]
     out = {
         "metric": "particle-steps/s (plain FP32 step, 3D bulk, 1M particles)",
         "value": value,
@@ -221,7 +227,7 @@ def main():
                    "particles": n, "bins": nb, "bin_width": w, "autotuned": tuned},
         "kernels_us": fam,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_pair32<3>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_pair32<3>",
                      "kernel_us": pair_us, "bytes_per_particle": PAIR_BYTES_PER_PARTICLE},
         "step_hbm_gbs": STEP_BYTES_PER_PARTICLE * value / 1e9,
         "timed_region": "steps launched directly with per-kernel HIP events (64-step chunks)",
