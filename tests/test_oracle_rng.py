@@ -63,6 +63,18 @@ def test_dsfmt_matches_reference_build(seed):
     assert np.array_equal(mine.view(np.uint64), ref.view(np.uint64))
 
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dsfmt_ref.npz")
+
+
+@pytest.mark.parametrize("seed", [1, 408, 4357, 0xDEADBEEF])
+def test_dsfmt_matches_reference_golden_vectors(seed):
+    """The same check against the committed vectors of the reference build
+    (tests/golden/make_dsfmt_golden.py), so it holds where the reference tree is absent."""
+    ref = np.load(GOLDEN)[f"seed_{seed}"]
+    mine = O.dsfmt_fill(seed, 0, len(ref)) + 1.0
+    assert np.array_equal(mine.view(np.uint64), ref)
+
+
 def _nint(a):
     return int(a + 0.5) if a >= 0.0 else int(a - 0.5)
 
