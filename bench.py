@@ -155,6 +155,9 @@ def main():
                     help="pin the bins per cut-off (skips the autotuner)")
     ap.add_argument("--block", type=int, default=0, help="pin the pair kernel block size (with --bins-per-cutoff)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-list", action="store_true", help="stencil pair kernel instead of the neighbour list")
+    ap.add_argument("--skin", type=float, default=0.3)
+    ap.add_argument("--list-max", type=int, default=160)
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--judge-every", type=int, default=16)
@@ -182,6 +185,9 @@ def main():
     if not args.bins_per_cutoff:
         tuned = md.autotune(reps=3)
         log(f"autotune: {tuned}")
+    if not args.no_list:
+        md.set_neighbour_list(True, args.skin, args.list_max)
+        log(f"neighbour list on: skin {args.skin} sigma, {args.list_max} entries per particle")
     ts = md.step(args.warmup, 0)
     md.set_timing(True)
     md.num_cells()  # synchronise
@@ -207,7 +213,7 @@ def main():
     achieved = PAIR_BYTES_PER_PARTICLE * n / (pair_us * 1e-6) / 1e9 if pair_us > 0 else 0.0
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "pair32_1M_hbm_traffic.json")
-    if n == N_PARTICLES and os.path.exists(tpath):  # FETCH/WRITE counter runs, copy-calibrated
+    if args.no_list and n == N_PARTICLES and os.path.exists(tpath):  # FETCH/WRITE counter runs, copy-calibrated
         traffic = json.load(open(tpath))["traffic_bytes_per_launch"]
     out = {
         "metric": "particle-steps/s (plain FP32 step, 3D bulk, 1M particles)",
@@ -227,7 +233,8 @@ def main():
                    "particles": n, "bins": nb, "bin_width": w, "autotuned": tuned},
         "kernels_us": fam,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_pair32<3>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_pair32<3>" if args.no_list else "k_pair_list<3>",
                      "kernel_us": pair_us, "bytes_per_particle": PAIR_BYTES_PER_PARTICLE},
         "step_hbm_gbs": STEP_BYTES_PER_PARTICLE * value / 1e9,
         "timed_region": "steps launched directly with per-kernel HIP events (64-step chunks)",

@@ -198,6 +198,10 @@ class HipMD:
     def set_kernel(self, variant):
         _check(self.L, self.ctx, self.L.cil_hip_set_kernel(self.ctx, int(variant)), "set_kernel")
 
+    def set_neighbour_list(self, on=True, skin=0.3, max_neighbours=160):
+        _check(self.L, self.ctx, self.L.cil_hip_set_neighbour_list(self.ctx, 1 if on else 0, float(skin),
+                                                                   int(max_neighbours)), "set_neighbour_list")
+
     def set_timing(self, on=True):
         _check(self.L, self.ctx, self.L.cil_hip_set_timing(self.ctx, 1 if on else 0), "set_timing")
 

@@ -35,6 +35,10 @@ using namespace cil;
 // has its own L2.  Kernels whose consecutive workgroups work on neighbouring bins renumber
 // them so that each XCD processes one contiguous range: a bin's neighbour data is then
 // fetched into one L2 instead of eight.  A bijection of [0, nblk).
+// Neighbour-list mode gates the sort kernels on the device rebuild flag (S.rebuild is null
+// outside list mode): steps that keep the list skip the whole re-sort with no host round trip.
+__device__ inline bool sort_skipped(const Dev& S) { return S.rebuild != nullptr && *S.rebuild == 0u; }
+
 #define NUM_XCD 8
 __device__ inline int xcd_block(int b, int nblk) {
   const int q = nblk / NUM_XCD, r = nblk % NUM_XCD, x = b % NUM_XCD;
@@ -57,6 +61,7 @@ __device__ inline int xcd_block_chunked(int b, int nblk) {
 // ======================================================================================
 __device__ inline void k_keys_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
   const StepParams& P = *Pp;
+  if (sort_skipped(S)) return;
   int p = bid * blockDim.x + threadIdx.x;
   if (p >= S.P) return;
   int s, c;
@@ -133,7 +138,11 @@ __device__ inline void k_scan_local_body(const int* __restrict__ in, int* __rest
   }
   if (threadIdx.x == 0) bsum[bid] = total;
 }
-__global__ void k_scan_local(const int* __restrict__ in, int* __restrict__ out, int* __restrict__ bsum, int n) { k_scan_local_body(in, out, bsum, n, (int)blockIdx.x); }
+__global__ void k_scan_local(const int* __restrict__ in, int* __restrict__ out, int* __restrict__ bsum, int n,
+                             const unsigned* gate) {
+  if (gate && *gate == 0u) return;
+  k_scan_local_body(in, out, bsum, n, (int)blockIdx.x);
+}
 
 __device__ inline void k_scan_top_body(int* __restrict__ bsum, int nb, int* __restrict__ total_out, int bid) {
   __shared__ int lds[WAVE + 1];
@@ -148,18 +157,25 @@ __device__ inline void k_scan_top_body(int* __restrict__ bsum, int nb, int* __re
   }
   if (threadIdx.x == 0) *total_out = carry;
 }
-__global__ void k_scan_top(int* __restrict__ bsum, int nb, int* __restrict__ total_out) { k_scan_top_body(bsum, nb, total_out, (int)blockIdx.x); }
+__global__ void k_scan_top(int* __restrict__ bsum, int nb, int* __restrict__ total_out, const unsigned* gate) {
+  if (gate && *gate == 0u) return;
+  k_scan_top_body(bsum, nb, total_out, (int)blockIdx.x);
+}
 
 __device__ inline void k_scan_add_body(int* __restrict__ out, const int* __restrict__ bsum, int n, int bid) {
   int i = bid * blockDim.x + threadIdx.x;
   if (i < n) out[i] += bsum[i / SCAN_TILE];
 }
-__global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ bsum, int n) { k_scan_add_body(out, bsum, n, (int)blockIdx.x); }
+__global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ bsum, int n, const unsigned* gate) {
+  if (gate && *gate == 0u) return;
+  k_scan_add_body(out, bsum, n, (int)blockIdx.x);
+}
 
 // ======================================================================================
 // Scatter and per-bin sort + gather
 // ======================================================================================
 __device__ inline void k_scatter_body(const Dev& S, int bid) {
+  if (sort_skipped(S)) return;
   int p = bid * blockDim.x + threadIdx.x;
   if (p >= S.P) return;
   int b = S.key[p];
@@ -214,6 +230,7 @@ __device__ inline void write_sorted<double>(const StepParams& P, const Dev& S, i
 template <typename T>
 __device__ inline void k_binsort_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
   const StepParams& P = *Pp;
+  if (sort_skipped(S)) return;
   __shared__ int buf[BINSORT_WAVES][BINSORT_MAX];
   int wid = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
   int u = bid * BINSORT_WAVES + wid;  // bucket (type, spatial bin)
@@ -688,6 +705,107 @@ __global__ void __launch_bounds__(256) k_pair(const StepParams* __restrict__ Pp,
 // wave-uniform (scalar registers).  Same traversal and pair semantics as k_pair<float>.
 // Single-disc cells: the same-cell exclusion is j == k, so the inner loop reads only the
 // 16-byte sorted record of each candidate.
+// Law coefficients of one (target, source) element-type pair for the FP32 kernels.
+struct Law32 {
+  int law;
+  bool shifted;
+  float rcut2, eps, xi, e0, ew, hzk, ewell_p, ewell_q, ea_p, ea_q;
+};
+__device__ inline Law32 law32(const StepParams& P, int tp, int tq) {
+  const LawEntry& A = P.law[tp][tq];
+  const LawEntry& B = P.law[tq][tp];
+  Law32 L;
+  L.law = A.law;
+  L.shifted = P.force_shifted != 0;
+  L.rcut2 = (float)A.rcut2;
+  L.eps = (float)A.eps;
+  L.xi = (float)A.xi;
+  L.e0 = (float)A.e0;
+  L.ew = (float)A.ew;
+  L.hzk = (float)A.hzk;
+  L.ewell_p = (float)A.ewell_i;
+  L.ewell_q = (float)B.ewell_i;
+  L.ea_p = (float)A.ea_ei;
+  L.ea_q = (float)B.ea_ei;
+  return L;
+}
+
+// One pair term of the FP32 kernels (LJParams / SCParams::elemForce, MD.cuh:89-110,190-207;
+// contact laws with the device semantics of MD.cu:317-362, canonical i = lower index).
+// Returns false when the pair adds nothing: outside the law's range, or an LJ pair inside the
+// 1e-5 cut-off guard band, which is recorded in gslot for the exact FP64 decision.  For a
+// coincident 2D contact (dx, dy) become the reference's random direction.
+template <int D>
+__device__ __forceinline__ bool pair_term32(const StepParams& P, const Dev& S, const Law32& L, int k, int j, int pidx,
+                                            float sp_, float sq_, float r2, uint32_t ts, float& dx, float& dy,
+                                            int& nguard, float& f0, bool& contact, float& temp) {
+  const float ssum = sq_ + sp_;
+  contact = false;
+  temp = 0.0f;
+  if (L.law == CIL_LAW_LJ) {  // LJParams::elemForce (MD.cuh:89-110)
+    const float sh = 0.5f * ssum;
+    const float sij2 = sh * sh;
+    const float rc2 = sij2 * L.rcut2;
+    if (r2 > rc2 * 1.00001f) return false;
+    if (r2 > rc2 * 0.99999f) {  // decided exactly in FP64 by k_lj_guard_fix
+      if (nguard < GUARD_SLOTS) S.gslot[(size_t)k * GUARD_SLOTS + nguard] = j;
+      ++nguard;
+      return false;
+    }
+    const float ir2 = 1.0f / r2;
+    const float s2 = sij2 * ir2;
+    const float s6 = s2 * s2 * s2;
+    float f = (s6 - 2.0f * s6 * s6) * ir2;
+    if (L.shifted) f -= lj_f0(sij2, rc2) * sqrtf(rc2 * ir2);
+    f0 = f * L.eps;
+  } else if (L.law == CIL_LAW_SC) {  // SCParams::elemForce (MD.cuh:190-207)
+    const float sigCore = 0.5f * ssum;
+    const float up = sigCore + L.xi, lo = sigCore - L.xi;
+    const float rc2 = L.rcut2 * up * up;
+    if (!(r2 > L.rcut2 * lo * lo && r2 <= rc2)) return false;
+    const float rr = sqrtf(r2);
+    const float y = rr - sigCore;
+    const float ixi3 = 6.0f / (L.xi * L.xi * L.xi);
+    if (y <= -L.xi) f0 = 0.0f;
+    else if (y < 0.0f) f0 = -(L.e0 + L.ew) * ((-y) * (y + L.xi) * ixi3) / rr;
+    else if (y < L.xi) f0 = L.ew * (y * (L.xi - y) * ixi3) / rr;
+    else f0 = 0.0f;
+    if (L.shifted) f0 -= sc_f0(sigCore, rc2, L.xi, L.e0, L.ew) * sqrtf(rc2 / r2);
+  } else {  // contact laws (device semantics MD.cu:317-362), canonical i = lower index
+    const int qidx = S.smeta[j] >> 3;
+    const bool p_is_i = pidx < qidx;
+    const float si = p_is_i ? sp_ : sq_, sj = p_is_i ? sq_ : sp_;
+    const float rr = sqrtf(r2);
+    if (!(rr <= (si + sj) * 0.5f)) return false;
+    contact = true;
+    if (D == 2 && dx == 0.0f && dy == 0.0f) {
+      double rij[2];
+      coincident_dir(P, p_is_i ? pidx : qidx, p_is_i ? qidx : pidx, ts, rij);
+      dx = (float)(p_is_i ? rij[0] : -rij[0]);
+      dy = (float)(p_is_i ? rij[1] : -rij[1]);
+    }
+    const float hh = (si + sj) * 0.5f - rr;
+    const float h = hh >= 0.0f ? hh : 0.0f;
+    const float ra = si * 0.5f, rb = sj * 0.5f;
+    if (L.law == CIL_LAW_HZ) f0 = -(h * sqrtf(h) / (L.hzk * sqrtf(1.0f / ra + 1.0f / rb)));
+    else f0 = -(h * (p_is_i ? L.ea_p : L.ea_q));
+    float Acon;
+    if (D == 3) {
+      const float uu = r2 - rb * rb + ra * ra;
+      const float t = sqrtf(4.f * r2 * ra * ra - uu * uu) / (2.f * rr);
+      Acon = t > 0.f ? (float)kPi * t * t : 0.f;
+    } else {
+      const float q = ra - (2.f * rb * h - h * h) / (2.f * (ra + rb - h));
+      const float t = 2.f * sqrtf(ra * ra - q * q);
+      Acon = isnan(t) ? (si <= sj ? si : sj) : t;
+    }
+    f0 += (p_is_i ? L.ewell_p : L.ewell_q) * Acon;
+    temp = (Acon == 0.f ? 0.f : (4.f * (float)kPi * ra * ra / 3.f));
+    temp = (D == 3 ? temp : Acon);
+  }
+  return true;
+}
+
 template <int D>
 __device__ inline void k_pair32_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
   const StepParams& P = *Pp;
@@ -732,13 +850,8 @@ __device__ inline void k_pair32_body(const StepParams* __restrict__ Pp, const De
     const int Rx = P.reach_tt[tp][tq][0];
     if (Rx < 0) continue;
     const int Ry = P.reach_tt[tp][tq][1], Rz = D == 3 ? P.reach_tt[tp][tq][2] : 0;
-    const LawEntry& A = P.law[tp][tq];
-    const LawEntry& B = P.law[tq][tp];
-    const int law = A.law;
+    const Law32 Lw = law32(P, tp, tq);
     const float rca = (float)P.rca_tt[tp][tq], rcb = (float)P.rcb_tt[tp][tq];
-    const float rcut2 = (float)A.rcut2, eps = (float)A.eps, xi = (float)A.xi, e0 = (float)A.e0, ew = (float)A.ew;
-    const float hzk = (float)A.hzk;
-    const float ewell_p = (float)A.ewell_i, ewell_q = (float)B.ewell_i, ea_p = (float)A.ea_ei, ea_q = (float)B.ea_ei;
     const float cadh = (tq / nE == spec_p) ? 1.0f : ca_p;  // scale for f0 >= 0 (MD.cu:345-360)
     const bool same_species_cells = nE == 2 && (tq / nE == spec_p);
     const bool self_bucket = tq == tp;
@@ -796,70 +909,9 @@ __device__ inline void k_pair32_body(const StepParams* __restrict__ Pp, const De
             if (r2 > rc * rc * 1.0001f) continue;
             if (self_bucket && j == k) continue;
             if (same_species_cells && ((smeta[j] >> 3) >> cshift) == cell_p) continue;
-            float f0;
-            bool contact = false;
-            float temp = 0.0f;
-            if (law == CIL_LAW_LJ) {  // LJParams::elemForce (MD.cuh:89-110)
-              const float sh = 0.5f * ssum;
-              const float sij2 = sh * sh;
-              const float rc2 = sij2 * rcut2;
-              if (r2 > rc2 * 1.00001f) continue;
-              if (r2 > rc2 * 0.99999f) {  // decided exactly in FP64 by k_lj_guard_fix
-                if (nguard < GUARD_SLOTS) S.gslot[(size_t)k * GUARD_SLOTS + nguard] = j;
-                ++nguard;
-                continue;
-              }
-              const float ir2 = 1.0f / r2;
-              const float s2 = sij2 * ir2;
-              const float s6 = s2 * s2 * s2;
-              float f = (s6 - 2.0f * s6 * s6) * ir2;
-              if (shifted) f -= lj_f0(sij2, rc2) * sqrtf(rc2 * ir2);
-              f0 = f * eps;
-            } else if (law == CIL_LAW_SC) {  // SCParams::elemForce (MD.cuh:190-207)
-              const float sigCore = 0.5f * ssum;
-              const float up = sigCore + xi, lo = sigCore - xi;
-              const float rc2 = rcut2 * up * up;
-              if (!(r2 > rcut2 * lo * lo && r2 <= rc2)) continue;
-              const float rr = sqrtf(r2);
-              const float y = rr - sigCore;
-              const float ixi3 = 6.0f / (xi * xi * xi);
-              if (y <= -xi) f0 = 0.0f;
-              else if (y < 0.0f) f0 = -(e0 + ew) * ((-y) * (y + xi) * ixi3) / rr;
-              else if (y < xi) f0 = ew * (y * (xi - y) * ixi3) / rr;
-              else f0 = 0.0f;
-              if (shifted) f0 -= sc_f0(sigCore, rc2, xi, e0, ew) * sqrtf(rc2 / r2);
-            } else {  // contact laws (device semantics MD.cu:317-362), canonical i = lower index
-              const int qidx = smeta[j] >> 3;
-              const bool p_is_i = pidx < qidx;
-              const float si = p_is_i ? me.w : r.w, sj = p_is_i ? r.w : me.w;
-              const float rr = sqrtf(r2);
-              if (!(rr <= (si + sj) * 0.5f)) continue;
-              contact = true;
-              if (D == 2 && dx == 0.0f && dy == 0.0f) {
-                double rij[2];
-                coincident_dir(P, p_is_i ? pidx : qidx, p_is_i ? qidx : pidx, ts, rij);
-                dx = (float)(p_is_i ? rij[0] : -rij[0]);
-                dy = (float)(p_is_i ? rij[1] : -rij[1]);
-              }
-              const float hh = (si + sj) * 0.5f - rr;
-              const float h = hh >= 0.0f ? hh : 0.0f;
-              const float ra = si * 0.5f, rb = sj * 0.5f;
-              if (law == CIL_LAW_HZ) f0 = -(h * sqrtf(h) / (hzk * sqrtf(1.0f / ra + 1.0f / rb)));
-              else f0 = -(h * (p_is_i ? ea_p : ea_q));
-              float Acon;
-              if (D == 3) {
-                const float uu = r2 - rb * rb + ra * ra;
-                const float t = sqrtf(4.f * r2 * ra * ra - uu * uu) / (2.f * rr);
-                Acon = t > 0.f ? (float)kPi * t * t : 0.f;
-              } else {
-                const float q = ra - (2.f * rb * h - h * h) / (2.f * (ra + rb - h));
-                const float t = 2.f * sqrtf(ra * ra - q * q);
-                Acon = isnan(t) ? (si <= sj ? si : sj) : t;
-              }
-              f0 += (p_is_i ? ewell_p : ewell_q) * Acon;
-              temp = (Acon == 0.f ? 0.f : (4.f * (float)kPi * ra * ra / 3.f));
-              temp = (D == 3 ? temp : Acon);
-            }
+            float f0, temp;
+            bool contact;
+            if (!pair_term32<D>(P, S, Lw, k, j, pidx, me.w, r.w, r2, ts, dx, dy, nguard, f0, contact, temp)) continue;
             const float fs = f0 < 0.0f ? f0 : cadh * f0;
             Fx = fmaf(fs, dx, Fx);
             Fy = fmaf(fs, dy, Fy);
@@ -884,6 +936,201 @@ __global__ void __launch_bounds__(256) k_pair32(const StepParams* __restrict__ P
 // LJ pairs whose FP32 distance fell within 1e-5 of the cut-off: exact FP64 decision
 // (LJParams::elemForce predicate r^2 <= RCUT^2 sigma_ij^2 on the canonical positions), then
 // the FP32 force added by the thread that owns the target particle (deterministic).
+// ======================================================================================
+// Neighbour-list mode (FP32).  A list entry is (sorted index << 10) | bin offset code; the
+// bins are sized for cut-off + skin, so the stencil reach is at most 4 and the offsets
+// (-4..4 per dimension) fit 10 bits.  Entries of one particle are grouped by source type.
+// ======================================================================================
+__device__ inline int list_code(int ox, int oy, int oz) { return ((ox + 4) * 9 + (oy + 4)) * 9 + (oz + 4); }
+
+// Wave -> (element type, sorted range) of the per-type wave layout used by k_pair32.
+__device__ inline bool wave_type_slot(const StepParams& P, const Dev& S, int bid, int* tp_out, int* k_out) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = __builtin_amdgcn_readfirstlane(bid * (blockDim.x / WAVE) + threadIdx.x / WAVE);
+  int tp = -1, k0 = 0, k1 = 0, acc = 0;
+  for (int t = 0; t < P.ntypes; ++t) {
+    const int T0 = S.start[t * P.nbins], T1 = S.start[(t + 1) * P.nbins];
+    const int nw = (T1 - T0 + WAVE - 1) / WAVE;
+    if (tp < 0 && w < acc + nw) {
+      tp = t;
+      k0 = T0 + (w - acc) * WAVE;
+      k1 = T1;
+    }
+    acc += nw;
+  }
+  if (tp < 0) return false;
+  *tp_out = __builtin_amdgcn_readfirstlane(tp);
+  *k_out = k0 + lane;
+  return k0 + lane < k1;
+}
+
+// List build (only when the rebuild flag is set): every pair within its own cut-off plus the
+// skin, found with the culled stencil walk of k_pair32; records the build positions.
+template <int D>
+__device__ inline void k_build_list_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
+  const StepParams& P = *Pp;
+  if (*S.rebuild == 0u) return;
+  int tp, k;
+  if (!wave_type_slot(P, S, bid, &tp, &k)) return;
+  const float4* __restrict__ spos = (const float4*)S.spos;
+  const int* __restrict__ start = S.start;
+  const int nE = P.nE, cshift = nE - 1, spec_p = tp / nE;
+  const int pidx = S.smeta[k] >> 3, cell_p = pidx >> cshift;
+  for (int d = 0; d < D; ++d) S.xref[(size_t)d * S.P + pidx] = (float)S.pos[(size_t)d * S.P + pidx];
+  int bc[3];
+  bin_coords(P, S.sbin[k], bc);
+  const float4 me = spos[k];
+  const float wx = (float)P.w[0], wy = (float)P.w[1], wz = (float)P.w[2];
+  const bool mix = P.reach[0] == 0 && P.box.pbc[0], miy = P.reach[1] == 0 && P.box.pbc[1];
+  const bool miz = D == 3 && P.reach[2] == 0 && P.box.pbc[2];
+  const float skin = P.skin;
+  const size_t NP = (size_t)S.P;
+  int cnt = 0;
+  bool over = false;
+  for (int tq = 0; tq < P.ntypes; ++tq) {
+    const int c0 = cnt;
+    const int Rx = P.reach_tt[tp][tq][0];
+    if (Rx >= 0) {
+      const int Ry = P.reach_tt[tp][tq][1], Rz = D == 3 ? P.reach_tt[tp][tq][2] : 0;
+      const float rca = (float)P.rca_tt[tp][tq], rcb = (float)P.rcb_tt[tp][tq];
+      const bool same_species_cells = nE == 2 && (tq / nE == spec_p);
+      const bool self_bucket = tq == tp;
+      const float rcp = (fmaf(rca, me.w + (float)P.sigma_bound[tq], rcb) + skin) * 1.001f + 1e-6f;
+      const float rcp2 = rcp * rcp;
+      int z0[2], z1[2], zo[2];
+      for (int ox = -Rx; ox <= Rx; ++ox) {
+        int nx = bc[0] + ox;
+        if (P.box.pbc[0]) nx = nx < 0 ? nx + P.ns[0] : (nx >= P.ns[0] ? nx - P.ns[0] : nx);
+        else if (nx < 0 || nx >= P.ns[0]) continue;
+        const float qx = me.x - (float)ox * wx;
+        const float gx = mix ? 0.0f : (ox > 0 ? (float)ox * wx - me.x : (ox < 0 ? me.x - (float)(ox + 1) * wx : 0.0f));
+        const float gx2 = gx > 0.0f ? gx * gx : 0.0f;
+        if (gx2 > rcp2) continue;
+        for (int oy = -Ry; oy <= Ry; ++oy) {
+          int ny = bc[1] + oy;
+          if (P.box.pbc[1]) ny = ny < 0 ? ny + P.ns[1] : (ny >= P.ns[1] ? ny - P.ns[1] : ny);
+          else if (ny < 0 || ny >= P.ns[1]) continue;
+          const float qy = me.y - (float)oy * wy;
+          const float gy = miy ? 0.0f : (oy > 0 ? (float)oy * wy - me.y : (oy < 0 ? me.y - (float)(oy + 1) * wy : 0.0f));
+          const float rem = rcp2 - gx2 - (gy > 0.0f ? gy * gy : 0.0f);
+          if (rem < 0.0f) continue;
+          int zlo = -Rz, zhi = Rz;
+          if (D == 3 && !miz) {
+            const float sr = sqrtf(rem);
+            const int a = (int)floorf((me.z + sr) / wz), b = (int)ceilf((me.z - sr) / wz) - 1;
+            zhi = a < Rz ? a : Rz;
+            zlo = b > -Rz ? b : -Rz;
+            if (zlo > zhi) continue;
+          }
+          const int nseg = z_segments_range(P, bc[2], zlo, zhi, z0, z1, zo);
+          for (int sg = 0; sg < nseg; ++sg) {
+            const int u0 = bucket_of(P, tq, nx, ny, z0[sg]);
+            const int u1 = bucket_of(P, tq, nx, ny, z1[sg]);
+            const int j1 = start[u1 + 1];
+            int u = u0, jend = start[u0 + 1];
+            float qz = me.z - (float)zo[sg] * wz;
+            for (int j = start[u0]; j < j1; ++j) {
+              while (j >= jend) {
+                ++u;
+                jend = start[u + 1];
+                qz -= wz;
+              }
+              const float4 r = spos[j];
+              float dx = r.x - qx, dy = r.y - qy, dz = D == 3 ? r.z - qz : 0.0f;
+              if (mix) dx -= (float)P.box.L[0] * rintf(dx * (float)P.box.invL[0]);
+              if (miy) dy -= (float)P.box.L[1] * rintf(dy * (float)P.box.invL[1]);
+              if (miz) dz -= (float)P.box.L[2] * rintf(dz * (float)P.box.invL[2]);
+              const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+              const float rl = (fmaf(rca, r.w + me.w, rcb) + skin) * 1.001f;
+              if (r2 > rl * rl) continue;
+              if (self_bucket && j == k) continue;
+              if (same_species_cells && ((S.smeta[j] >> 3) >> cshift) == cell_p) continue;
+              const int oz = D == 3 ? zo[sg] + (u - u0) : 0;
+              if (cnt < P.list_max)
+                S.nl[(size_t)cnt * NP + k] =
+                    (int)(((unsigned)j << 10) | (unsigned)list_code(mix ? 0 : ox, miy ? 0 : oy, miz ? 0 : oz));
+              else over = true;
+              ++cnt;
+            }
+          }
+        }
+      }
+    }
+    S.nlt[(size_t)tq * NP + k] = (cnt < P.list_max ? cnt : P.list_max) - (c0 < P.list_max ? c0 : P.list_max);
+  }
+  if (over) atomicOr(S.err, (unsigned)ERR_LIST);
+}
+template <int D>
+__global__ void __launch_bounds__(256) k_build_list(const StepParams* __restrict__ Pp, Dev S) {
+  k_build_list_body<D>(Pp, S, xcd_block_chunked(blockIdx.x, gridDim.x));
+}
+
+// Pair forces over the neighbour list (every step).  Same law code, exclusions, guard band
+// and per-type wave layout as k_pair32; clears the rebuild flag for the next step.
+template <int D>
+__device__ inline void k_pair_list_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
+  const StepParams& P = *Pp;
+  if (bid == 0 && threadIdx.x == 0) *S.rebuild = 0u;
+  int tp, k;
+  if (!wave_type_slot(P, S, bid, &tp, &k)) return;
+  const float4* __restrict__ spos = (const float4*)S.spos;
+  const uint32_t ts = (uint32_t)*S.ts;
+  const int nE = P.nE, spec_p = tp / nE;
+  const float ca_p = (float)P.cross_adh[spec_p];
+  const int pidx = S.smeta[k] >> 3;
+  const float4 me = spos[k];
+  const float wx = (float)P.w[0], wy = (float)P.w[1], wz = (float)P.w[2];
+  const bool mix = P.reach[0] == 0 && P.box.pbc[0], miy = P.reach[1] == 0 && P.box.pbc[1];
+  const bool miz = D == 3 && P.reach[2] == 0 && P.box.pbc[2];
+  const size_t NP = (size_t)S.P;
+  float Fx = 0.f, Fy = 0.f, Fz = 0.f, pr = 0.f;
+  int nguard = 0, m = 0;
+  for (int tq = 0; tq < P.ntypes; ++tq) {
+    const int n = S.nlt[(size_t)tq * NP + k];
+    if (n == 0) continue;
+    const Law32 Lw = law32(P, tp, tq);
+    const float rca = (float)P.rca_tt[tp][tq], rcb = (float)P.rcb_tt[tp][tq];
+    const float cadh = (tq / nE == spec_p) ? 1.0f : ca_p;  // scale for f0 >= 0 (MD.cu:345-360)
+    for (const int m1 = m + n; m < m1; ++m) {
+      const unsigned e = (unsigned)S.nl[(size_t)m * NP + k];
+      const int j = (int)(e >> 10), code = (int)(e & 1023u);
+      const float4 r = spos[j];
+      float dx = r.x - (me.x - (float)(code / 81 - 4) * wx);
+      float dy = r.y - (me.y - (float)((code / 9) % 9 - 4) * wy);
+      float dz = D == 3 ? r.z - (me.z - (float)(code % 9 - 4) * wz) : 0.0f;
+      if (mix) dx -= (float)P.box.L[0] * rintf(dx * (float)P.box.invL[0]);
+      if (miy) dy -= (float)P.box.L[1] * rintf(dy * (float)P.box.invL[1]);
+      if (miz) dz -= (float)P.box.L[2] * rintf(dz * (float)P.box.invL[2]);
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      const float rc = fmaf(rca, r.w + me.w, rcb);
+      if (r2 > rc * rc * 1.0001f) continue;
+      float f0, temp;
+      bool contact;
+      if (!pair_term32<D>(P, S, Lw, k, j, pidx, me.w, r.w, r2, ts, dx, dy, nguard, f0, contact, temp)) continue;
+      const float fs = f0 < 0.0f ? f0 : cadh * f0;
+      Fx = fmaf(fs, dx, Fx);
+      Fy = fmaf(fs, dy, Fy);
+      if (D == 3) Fz = fmaf(fs, dz, Fz);
+      if (contact && temp != 0.0f) pr -= fs / temp;
+    }
+  }
+  float* fp = (float*)S.fpair;
+  fp[k] = Fx;
+  fp[NP + k] = Fy;
+  if (D == 3) fp[2 * NP + k] = Fz;
+  ((float*)S.ppair)[k] = P.pressure ? pr : 0.0f;
+  S.gcount[k] = nguard;
+  if (nguard > GUARD_SLOTS) atomicOr(S.err, (unsigned)ERR_GUARD);
+}
+template <int D>
+__global__ void __launch_bounds__(256) k_pair_list(const StepParams* __restrict__ Pp, Dev S) {
+  k_pair_list_body<D>(Pp, S, xcd_block_chunked(blockIdx.x, gridDim.x));
+}
+
+__global__ void k_set_flag(unsigned* flag, unsigned v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *flag = v;
+}
+
 __device__ inline void k_lj_guard_fix_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
   const StepParams& P = *Pp;
   const int total = S.start[P.nbuckets];
@@ -1093,7 +1340,7 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
       S.pos[(size_t)d * S.P + p] = xn[d];
       S.vel[(size_t)d * S.P + p] = F[e][d] * mu;
     }
-    if (emit_keys) {
+    if (emit_keys == 1) {
       bool finite = true;
       for (int d = 0; d < D; ++d) finite = finite && isfinite(xn[d]);
       if (!finite) {
@@ -1103,6 +1350,37 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
       int b = bucket_key(P, (int)p, s, xn);
       S.key[p] = b;
       S.rank[p] = atomicAdd(&S.count[b], 1);
+    } else if (emit_keys == 2) {
+      // neighbour-list mode: skin test against the build positions, then the bin-local FP32
+      // coordinates of the particle's sorted slot (bins of the last sort).  A set rebuild flag
+      // means this force pass re-sorts anyway (and, right after an upload, that no sorted slot
+      // exists yet), so the refresh is skipped then.
+      bool finite = true;
+      float dr2 = 0.0f;
+      for (int d = 0; d < D; ++d) {
+        finite = finite && isfinite(xn[d]);
+        float dd = (float)(xn[d] - (double)S.xref[(size_t)d * S.P + p]);
+        if (P.box.pbc[d]) dd -= (float)P.box.L[d] * rintf(dd * (float)P.box.invL[d]);
+        dr2 = fmaf(dd, dd, dr2);
+      }
+      if (!finite) atomicOr(S.err, (unsigned)ERR_NONFINITE);
+      if (!(dr2 <= 0.25f * P.skin * P.skin)) atomicOr(S.rebuild, 1u);
+      else if (*S.rebuild == 0u) {
+        const int k = S.inv[p];
+        int bc[3];
+        bin_coords(P, S.sbin[k], bc);
+        float4* sp4 = (float4*)S.spos;
+        float4 r = sp4[k];
+        double l[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < D; ++d) {
+          l[d] = xn[d] - bc[d] * P.w[d];
+          if (P.box.pbc[d]) l[d] -= P.box.L[d] * rint(l[d] * P.box.invL[d]);
+        }
+        r.x = (float)l[0];
+        r.y = (float)l[1];
+        r.z = D == 3 ? (float)l[2] : 0.0f;
+        sp4[k] = r;
+      }
     }
   }
 }

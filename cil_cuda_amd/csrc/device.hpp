@@ -22,7 +22,7 @@ enum CellStateV : int { CS_DIVIDING = 0, CS_CRAWLING = 1, CS_UNDEFINED = 2 };  /
 // Error bits raised by kernels and collected by the host at synchronisation points.
 enum ErrBit : unsigned { ERR_BIN_OVERFLOW = 1u, ERR_CAPACITY = 2u, ERR_BOND = 4u, ERR_NONFINITE = 8u,
                          ERR_CUTOFF = 16u, ERR_NOISE = 32u, ERR_GUARD = 64u,
-                         ERR_KNN = 128u };
+                         ERR_KNN = 128u, ERR_LIST = 256u };
 
 // ---------------------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11).  Counter = (id, ts, stream, k), key = seed.
@@ -155,6 +155,11 @@ struct StepParams {
   uint64_t seed;
   double osmotic_pi;
   double max_disp;
+  // neighbour-list mode (FP32): the list holds every pair within its cut-off + skin at the
+  // last build; it is rebuilt once any particle has moved more than skin / 2 since then
+  int list_mode;
+  int list_max;     // list capacity per particle
+  float skin;
   cil_wall walls[CIL_MAX_WALLS];
   cil_species sp[CIL_MAX_SPECIES];
   // bins: spatial grid ns[0] x ns[1] x ns[2]; particles are counting-sorted into buckets

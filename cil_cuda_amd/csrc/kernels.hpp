@@ -61,6 +61,11 @@ struct Dev {
   int* dscan;
   int* dblock;
   int* nnew;        // [CIL_MAX_SPECIES]
+  // neighbour-list mode
+  unsigned* rebuild;  // [1] nonzero: re-sort and rebuild the list in this force pass
+  int* nl;            // [list_max][P] (sorted index << 9 | bin offset code), per type contiguous
+  int* nlt;           // [ntypes][P] entries per source type
+  float* xref;        // [3][P] positions at the last build (particle order)
 };
 
 // ---------------------------------------------------------------------------------------

@@ -251,6 +251,14 @@ int cil_hip_autotune(cil_ctx* ctx, int reps, int* best_bins, int* best_block, in
 /* Per-kernel-family HIP-event timing (project addition; off by default). */
 int cil_hip_set_timing(cil_ctx* ctx, int on);
 
+/* Neighbour-list mode (project addition; FP32 force precision, fewer than 2^22 particle slots).
+ * Pairs within their cut-off + skin are listed at a build; each step evaluates only the list
+ * and re-sorts / rebuilds when a particle has moved more than skin / 2 since the build, when
+ * cells divide or are removed, or after an upload.  The pair set equals the stencil kernel's
+ * (FP32 summation order differs).  on = 0 returns to the stencil kernels.  Bins are sized for
+ * cut-off + skin while it is on; cil_hip_autotune requires it off. */
+int cil_hip_set_neighbour_list(cil_ctx* ctx, int on, double skin, int max_neighbours);
+
 /* sizeof the ABI structs (0: cil_params, 1: cil_species, 2: cil_species_view, 3: cil_box,
  * 4: cil_surrogate_stats),
  * so an FFI binding can verify its layout (project addition). */

@@ -1,0 +1,179 @@
+"""GPU parity of the neighbour-list mode (FP32) through the C ABI.
+
+Bars (the FP32 force bar of test_gpu_parity: |F - F_ref| <= 1e-5 (S_p + |F_ref| + 1 % of the
+median S_p) against the FP64 oracle):
+  - forces with the list built on the spot, for every law, 3D periodic and open boxes, one and
+    two discs per cell, including a box small enough for single-bin minimum-image dimensions;
+  - forces of steps that reuse a stale list (particles up to skin / 2 outside their
+    build-time bins) against the FP64 oracle;
+  - trajectories: over 300 noisy steps with a small skin (frequent rebuilds) the list mode,
+    the FP32 stencil kernel and an FP64 run agree within 1e-4 sigma (FP32 rounding, amplified
+    by the discontinuous LJ cut-off), and the forces of the end state still meet the bar;
+  - division and drain: growing channels give the same division events and cell counts as
+    the stencil kernel, positions within 1e-5 sigma.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from cil_cuda_amd import abi, configs
+from cil_cuda_amd.colony import HipMD
+from oracle import oracle as O
+
+sys.path.insert(0, __file__.rsplit("/", 1)[0])
+from test_gpu_parity import _max_disp_err, _random_colony  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _force_bar(p, box, col, frc, tol=1e-5):
+    orc = O.Oracle(p, box, traversal=1)
+    ref = col.copy()
+    orc.forces(ref, 0)
+    scale = orc.force_scale(ref)
+    fref = ref.flat("frc").reshape(-1, col.dim)
+    err = np.linalg.norm(frc - fref, axis=1)
+    bound = tol * (scale + np.linalg.norm(fref, axis=1) + 0.01 * float(np.median(scale))) + 1e-300
+    return float(np.max(err / bound))
+
+
+def _list_forces(p, box, col, skin=0.3, max_nb=256):
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    md.set_neighbour_list(True, skin, max_nb)
+    frc, _ = md.forces()
+    md.close()
+    return frc
+
+
+@pytest.mark.parametrize("law", [abi.LAW_LJ, abi.LAW_SC, abi.LAW_HZ])
+@pytest.mark.parametrize("periodic", [1, 0])
+@pytest.mark.parametrize("nelem", [1, 2])
+def test_list_forces_match_fp64(gpu, law, periodic, nelem):
+    say("list forces law", law, "periodic", periodic, "nelem", nelem)
+    p, box, col = _random_colony(3, nelem, 800, 9.0, periodic, law, seed=31 + law)
+    p.precision = abi.PREC_FP32
+    worst = _force_bar(p, box, col, _list_forces(p, box, col))
+    assert worst <= 1.0, worst
+
+
+def test_list_forces_small_periodic_box(gpu):
+    # a box of fewer than 2 * reach + 1 bins per dimension: single-bin minimum-image path
+    p, box, col = _random_colony(3, 1, 120, 4.5, 1, abi.LAW_LJ, seed=5)
+    p.precision = abi.PREC_FP32
+    worst = _force_bar(p, box, col, _list_forces(p, box, col))
+    assert worst <= 1.0, worst
+
+
+def _relaxed(n, seed):
+    p, box, col = configs.relaxed_bulk_3d(n, seed=seed, precision=abi.PREC_FP32)
+    return p, box, col
+
+
+def test_list_trajectory_matches_stencil_with_rebuilds(gpu):
+    """300 noisy steps with skin 0.02 (a rebuild every few steps): the list mode, the FP32
+    stencil kernel and an FP64 run stay within 1e-4 sigma of each other.  FP32 paths differ by
+    rounding (~1e-8 sigma per step); the unshifted LJ cut-off turns a pair that crosses rc one
+    step earlier in one run into a ~5e-6 sigma jump, so FP32 trajectories agree only to that
+    order (the 1e-9 sigma trajectory bar is the FP64 mode's).  The end-state forces still meet
+    the force bar."""
+    p, box, col = _relaxed(8192, 4)
+    out = {}
+    for mode in ("fp64", "stencil32", "list32"):
+        q = abi.Params.from_buffer_copy(p)
+        q.precision = abi.PREC_FP64 if mode == "fp64" else abi.PREC_FP32
+        md = HipMD(q, box, [sp["capacity"] for sp in col.species])
+        md.upload(col)
+        if mode == "list32":
+            md.set_neighbour_list(True, 0.02, 256)
+        md.forces(want=False)
+        assert md.step(300, 0) == 300
+        c = col.copy()
+        md.download(c)
+        md.close()
+        out[mode] = c
+    e_stencil = _max_disp_err(out["fp64"], out["stencil32"], box)
+    e_list = _max_disp_err(out["fp64"], out["list32"], box)
+    e_pair = _max_disp_err(out["stencil32"], out["list32"], box)
+    say(f"after 300 steps, max displacement vs FP64: stencil {e_stencil:.3e}, list {e_list:.3e}; "
+        f"list vs stencil {e_pair:.3e}")
+    assert max(e_stencil, e_list, e_pair) <= 1e-4
+    end = out["list32"]
+    worst = _force_bar(p, box, end, _list_forces(p, box, end))
+    assert worst <= 1.0, worst
+
+
+@pytest.mark.parametrize("skin,nsteps", [(0.3, 20), (0.02, 37)])
+def test_stale_list_forces_match_fp64(gpu, skin, nsteps):
+    """Forces of the last step's list pass (no fresh build: particles up to skin / 2 outside
+    their build-time bins, coordinates refreshed in place) against the FP64 oracle on the
+    downloaded state; download materialises those lazy forces without a new force pass."""
+    p, box, col = _relaxed(8192, 4)
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    md.set_neighbour_list(True, skin, 256)
+    md.forces(want=False)
+    md.step(nsteps, 0)
+    c = col.copy()
+    md.download(c)
+    md.close()
+    worst = _force_bar(p, box, c, c.flat("frc").reshape(-1, 3))
+    say(f"stale list (skin {skin}, {nsteps} steps): worst {worst:.3f}")
+    assert worst <= 1.0, worst
+
+
+def test_list_graph_and_eager_paths_agree(gpu):
+    # 64-step graph replays and single eager steps must give the same states
+    p, box, col = _relaxed(4096, 6)
+    res = []
+    for chunks in ((64, 64), (1,) * 128):
+        md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+        md.upload(col)
+        md.set_neighbour_list(True, 0.05, 256)
+        md.forces(want=False)
+        ts = 0
+        for n in chunks:
+            ts = md.step(n, ts)
+        c = col.copy()
+        md.download(c)
+        md.close()
+        res.append(c.flat("pos"))
+    assert np.array_equal(res[0], res[1])
+
+
+def test_list_division_and_drain_match_stencil(gpu):
+    p, box, col = configs.channel_2d_hz(width=12.0, length=30.0, capacity=512, dt=1e-3, tau=0.05,
+                                        precision=abi.PREC_FP32)
+    out = []
+    for use_list in (False, True):
+        md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+        md.upload(col)
+        if use_list:
+            md.set_neighbour_list(True, 0.1, 128)
+        md.forces(want=False)
+        ts = 0
+        for _ in range(6):
+            ts = md.step(50, ts)
+            md.check_positions()
+        c = col.copy()
+        md.download(c)
+        md.close()
+        out.append(c)
+    say("channel cells", out[0].ncells(), out[1].ncells())
+    assert out[0].ncells() == out[1].ncells()
+    assert out[0].ncells()[0] > 8
+    assert _max_disp_err(out[0], out[1], box) <= 1e-5
+
+
+def test_list_mode_rejects_fp64(gpu):
+    p, box, col = configs.plain_2d_sc(n_cells=50, L=12.0, seed=3)
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    with pytest.raises(abi.CilError):
+        md.set_neighbour_list(True)
+    md.close()
