@@ -1253,11 +1253,11 @@ __global__ void k_noise(const StepParams* __restrict__ Pp, Dev S) { k_noise_body
 // ======================================================================================
 // Forces of one cell at the last force evaluation: intra (lazy) + pair (sorted).
 // ======================================================================================
-template <typename T>
+template <typename T, int DIM, int NE>
 __device__ inline void cell_total_force(const StepParams& P, const Dev& S, int s, int c, int cslot, bool materialized,
                                         double pos[2][3], double vel[2][3], const double* sig, double F[2][3],
                                         double* pc, bool* pos1_changed) {
-  const int D = P.box.dim, nE = P.nE;
+  constexpr int D = DIM, nE = NE;
   *pos1_changed = false;
   if (materialized) {
     for (int e = 0; e < nE; ++e)
@@ -1265,7 +1265,7 @@ __device__ inline void cell_total_force(const StepParams& P, const Dev& S, int s
     *pc = S.pressure[cslot];
     return;
   }
-  intra_cell(P, S, s, c, cslot, *S.ts, pos, vel, sig, F, pos1_changed);
+  intra_cell<DIM, NE>(P, S, s, c, cslot, *S.ts, pos, vel, sig, F, pos1_changed);
   const T* fp = (const T*)S.fpair;
   double pr = 0.0;
   for (int e = 0; e < nE; ++e) {
@@ -1279,7 +1279,7 @@ __device__ inline void cell_total_force(const StepParams& P, const Dev& S, int s
 // ======================================================================================
 // Fused update (updateVelocityPositions, CIL.cu:902-939), one thread per cell slot.
 // ======================================================================================
-template <typename T>
+template <typename T, int DIM, int NE>
 __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const Dev& S, int materialized, int emit_keys, int bid) {
   const StepParams& P = *Pp;
   int cslot = bid * blockDim.x + threadIdx.x;
@@ -1290,7 +1290,7 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
       for (int e = 0; e < P.nE; ++e) S.key[(size_t)cslot * P.nE + e] = -1;
     return;
   }
-  const int D = P.box.dim, nE = P.nE;
+  constexpr int D = DIM, nE = NE;
   const cil_species& sp = P.sp[s];
   double pos[2][3], vel[2][3], sig[2], F[2][3];
   for (int e = 0; e < nE; ++e) {
@@ -1303,7 +1303,7 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
   }
   double pc;
   bool pos1_changed;
-  cell_total_force<T>(P, S, s, c, cslot, materialized != 0, pos, vel, sig, F, &pc, &pos1_changed);
+  cell_total_force<T, DIM, NE>(P, S, s, c, cslot, materialized != 0, pos, vel, sig, F, &pc, &pos1_changed);
   if (!materialized && P.pressure) S.pressure[cslot] = pc;
   const int tsu = *S.ts;  // step index before the increment (FDT noise key)
   double area = sigma2area(sig[0]);
@@ -1369,34 +1369,31 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
         const int k = S.inv[p];
         int bc[3];
         bin_coords(P, S.sbin[k], bc);
-        float4* sp4 = (float4*)S.spos;
-        float4 r = sp4[k];
-        double l[3] = {0.0, 0.0, 0.0};
+        float* sp = (float*)S.spos + 4 * (size_t)k;  // x, y, z of the float4 slot; sigma stays
         for (int d = 0; d < D; ++d) {
-          l[d] = xn[d] - bc[d] * P.w[d];
-          if (P.box.pbc[d]) l[d] -= P.box.L[d] * rint(l[d] * P.box.invL[d]);
+          double l = xn[d] - bc[d] * P.w[d];
+          if (P.box.pbc[d]) l -= P.box.L[d] * rint(l * P.box.invL[d]);
+          sp[d] = (float)l;
         }
-        r.x = (float)l[0];
-        r.y = (float)l[1];
-        r.z = D == 3 ? (float)l[2] : 0.0f;
-        sp4[k] = r;
       }
     }
   }
 }
-template <typename T>
+template <typename T, int DIM, int NE>
 __global__ void __launch_bounds__(256) k_update(const StepParams* __restrict__ Pp, Dev S, int materialized,
-                                                int emit_keys) { k_update_body<T>(Pp, S, materialized, emit_keys, (int)blockIdx.x); }
+                                                int emit_keys) {
+  k_update_body<T, DIM, NE>(Pp, S, materialized, emit_keys, (int)blockIdx.x);
+}
 
 // Forces at the current state into the canonical FP64 frc array (for download / parity).
-template <typename T>
+template <typename T, int DIM, int NE>
 __global__ void k_materialize(const StepParams* __restrict__ Pp, Dev S) {
   const StepParams& P = *Pp;
   int cslot = blockIdx.x * blockDim.x + threadIdx.x;
   if (cslot >= S.C) return;
   int s, c;
   if (!cell_active(P, S.ncells, cslot, &s, &c)) return;
-  const int D = P.box.dim, nE = P.nE;
+  constexpr int D = DIM, nE = NE;
   double pos[2][3], vel[2][3], sig[2], F[2][3], pc;
   for (int e = 0; e < nE; ++e) {
     size_t p = (size_t)cslot * nE + e;
@@ -1407,7 +1404,7 @@ __global__ void k_materialize(const StepParams* __restrict__ Pp, Dev S) {
     }
   }
   bool pos1_changed;
-  cell_total_force<T>(P, S, s, c, cslot, false, pos, vel, sig, F, &pc, &pos1_changed);
+  cell_total_force<T, DIM, NE>(P, S, s, c, cslot, false, pos, vel, sig, F, &pc, &pos1_changed);
   for (int e = 0; e < nE; ++e)
     for (int d = 0; d < D; ++d) S.frc[(size_t)d * S.P + (size_t)cslot * nE + e] = F[e][d];
   if (pos1_changed)

@@ -128,11 +128,14 @@ __device__ inline void on_sphere_counter(uint64_t seed, uint32_t id, uint32_t ts
   for (int d = 0; d < D; ++d) n[d] *= norm;
 }
 
+// DIM, NE: box dimension and discs per cell (compile-time, so the per-disc arrays stay in
+// registers); they must equal P.box.dim and P.nE.
+template <int DIM, int NE>
 __device__ inline void intra_cell(const StepParams& P, const Dev& S, int s, int c, int cslot, int ts,
                                   double pos[2][3], double vel[2][3], const double* sig, double F[2][3],
                                   bool* pos1_changed) {
   const cil_species& sp = P.sp[s];
-  const int D = P.box.dim, nE = P.nE;
+  constexpr int D = DIM, nE = NE;
   *pos1_changed = false;
   if (nE == 2) {
     double rij[3] = {0, 0, 0}, nij[3] = {0, 0, 0}, rnd[3] = {0, 0, 0};
