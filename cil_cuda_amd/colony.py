@@ -202,8 +202,14 @@ class HipMD:
         _check(self.L, self.ctx, self.L.cil_hip_set_neighbour_list(self.ctx, 1 if on else 0, float(skin),
                                                                    int(max_neighbours)), "set_neighbour_list")
 
-    def set_timing(self, on=True):
-        _check(self.L, self.ctx, self.L.cil_hip_set_timing(self.ctx, 1 if on else 0), "set_timing")
+    FAMILIES = ("update", "division", "sort", "pair", "noise", "guard", "build", "knn", "mlp", "macro")
+
+    def set_timing(self, on=True, families=None):
+        """Per-family HIP-event timing; `families` (names) restricts it, None times all."""
+        v = 0
+        if on:
+            v = 1 if families is None else sum(2 << self.FAMILIES.index(f) for f in families)
+        _check(self.L, self.ctx, self.L.cil_hip_set_timing(self.ctx, v), "set_timing")
 
     def timings(self):
         us = (C.c_double * 16)()

@@ -1091,10 +1091,9 @@ __device__ inline void k_pair_list_body(const StepParams* __restrict__ Pp, const
     const Law32 Lw = law32(P, tp, tq);
     const float rca = (float)P.rca_tt[tp][tq], rcb = (float)P.rcb_tt[tp][tq];
     const float cadh = (tq / nE == spec_p) ? 1.0f : ca_p;  // scale for f0 >= 0 (MD.cu:345-360)
-    for (const int m1 = m + n; m < m1; ++m) {
-      const unsigned e = (unsigned)S.nl[(size_t)m * NP + k];
+    // one list entry, in list order (the summation order is the list order)
+    auto term = [&](unsigned e, const float4& r) {
       const int j = (int)(e >> 10), code = (int)(e & 1023u);
-      const float4 r = spos[j];
       float dx = r.x - (me.x - (float)(code / 81 - 4) * wx);
       float dy = r.y - (me.y - (float)((code / 9) % 9 - 4) * wy);
       float dz = D == 3 ? r.z - (me.z - (float)(code % 9 - 4) * wz) : 0.0f;
@@ -1103,15 +1102,32 @@ __device__ inline void k_pair_list_body(const StepParams* __restrict__ Pp, const
       if (miz) dz -= (float)P.box.L[2] * rintf(dz * (float)P.box.invL[2]);
       const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
       const float rc = fmaf(rca, r.w + me.w, rcb);
-      if (r2 > rc * rc * 1.0001f) continue;
+      if (r2 > rc * rc * 1.0001f) return;
       float f0, temp;
       bool contact;
-      if (!pair_term32<D>(P, S, Lw, k, j, pidx, me.w, r.w, r2, ts, dx, dy, nguard, f0, contact, temp)) continue;
+      if (!pair_term32<D>(P, S, Lw, k, j, pidx, me.w, r.w, r2, ts, dx, dy, nguard, f0, contact, temp)) return;
       const float fs = f0 < 0.0f ? f0 : cadh * f0;
       Fx = fmaf(fs, dx, Fx);
       Fy = fmaf(fs, dy, Fy);
       if (D == 3) Fz = fmaf(fs, dz, Fz);
       if (contact && temp != 0.0f) pr -= fs / temp;
+    };
+    const int m1 = m + n;
+    // four entries per round: their list words and positions are loaded before any is used,
+    // so four independent gathers are in flight per lane
+    for (; m + 4 <= m1; m += 4) {
+      unsigned e[4];
+      float4 r[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) e[u] = (unsigned)S.nl[(size_t)(m + u) * NP + k];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) r[u] = spos[e[u] >> 10];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) term(e[u], r[u]);
+    }
+    for (; m < m1; ++m) {
+      const unsigned e = (unsigned)S.nl[(size_t)m * NP + k];
+      term(e, spos[e >> 10]);
     }
   }
   float* fp = (float*)S.fpair;

@@ -248,7 +248,11 @@ int cil_hip_set_kernel(cil_ctx* ctx, int variant);
  * computed by cil_hip_forces or a step (the state is not advanced). */
 int cil_hip_autotune(cil_ctx* ctx, int reps, int* best_bins, int* best_block, int* best_variant, double* best_us);
 
-/* Per-kernel-family HIP-event timing (project addition; off by default). */
+/* Per-kernel-family HIP-event timing (project addition; off by default).  on = 1 times every
+ * family; any other nonzero value is a mask with bit f+1 set for family f, in the order of
+ * the names cil_hip_timings returns (update, division, sort, pair, noise, guard, build, knn,
+ * mlp, macro).  Timed steps are launched directly (no graph): each timed family adds two
+ * event records per step. */
 int cil_hip_set_timing(cil_ctx* ctx, int on);
 
 /* Neighbour-list mode (project addition; FP32 force precision, fewer than 2^22 particle slots).
