@@ -217,8 +217,8 @@ def main():
     pair_us = fam.get("pair", 0.0)
     achieved = PAIR_BYTES_PER_PARTICLE * n / (pair_us * 1e-6) / 1e9 if pair_us > 0 else 0.0
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "pair32_1M_hbm_traffic.json")
-    if args.no_list and n == N_PARTICLES and os.path.exists(tpath):  # FETCH/WRITE counter runs, copy-calibrated
+    tpath = os.path.join(ROOT, "profiles", ("pair32" if args.no_list else "pair_list") + "_1M_hbm_traffic.json")
+    if n == N_PARTICLES and os.path.exists(tpath):  # FETCH/WRITE counter runs, copy-calibrated
         traffic = json.load(open(tpath))["traffic_bytes_per_launch"]
     out = {
         "metric": "particle-steps/s (plain FP32 step, 3D bulk, 1M particles)",

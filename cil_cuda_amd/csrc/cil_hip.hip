@@ -596,7 +596,7 @@ __device__ inline void k_pair_body(const StepParams* __restrict__ Pp, const Dev&
   const int total = S.start[P.nbuckets];
   const int k = bid * blockDim.x + threadIdx.x;
   if (k >= total) return;
-  const uint32_t ts = (uint32_t)*S.ts;
+  const uint32_t ts = (uint32_t)step_ts(S);
   const int meta = S.smeta[k];
   const int pidx = meta >> 3, tp = meta & 7;
   const int cshift = P.nE - 1;  // cell = particle >> cshift (nE is 1 or 2)
@@ -709,7 +709,7 @@ __global__ void __launch_bounds__(256) k_pair(const StepParams* __restrict__ Pp,
 struct Law32 {
   int law;
   bool shifted;
-  float rcut2, eps, xi, e0, ew, hzk, ewell_p, ewell_q, ea_p, ea_q;
+  float rcut2, eps, xi, ixi3, e0, ew, hzk, ewell_p, ewell_q, ea_p, ea_q;
 };
 __device__ inline Law32 law32(const StepParams& P, int tp, int tq) {
   const LawEntry& A = P.law[tp][tq];
@@ -720,6 +720,7 @@ __device__ inline Law32 law32(const StepParams& P, int tp, int tq) {
   L.rcut2 = (float)A.rcut2;
   L.eps = (float)A.eps;
   L.xi = (float)A.xi;
+  L.ixi3 = 6.0f / (L.xi * L.xi * L.xi);
   L.e0 = (float)A.e0;
   L.ew = (float)A.ew;
   L.hzk = (float)A.hzk;
@@ -730,53 +731,65 @@ __device__ inline Law32 law32(const StepParams& P, int tp, int tq) {
   return L;
 }
 
+// Hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32, 1 ulp) for the FP32 laws: the
+// FP32 force bar (1e-5 relative to the force scale) is far above their error, and each
+// replaces a ~10-instruction correctly rounded sequence.
+__device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt32(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// Law class of pair_term32: LC_ANY dispatches on the (wave-uniform) law at run time; a fixed
+// class compiles one law only (LC_CONTACT: Hertz or elastic adhesion, told apart at run time).
+constexpr int LC_ANY = -1, LC_CONTACT = 100;
+
 // One pair term of the FP32 kernels (LJParams / SCParams::elemForce, MD.cuh:89-110,190-207;
 // contact laws with the device semantics of MD.cu:317-362, canonical i = lower index).
 // Returns false when the pair adds nothing: outside the law's range, or an LJ pair inside the
 // 1e-5 cut-off guard band, which is recorded in gslot for the exact FP64 decision.  For a
 // coincident 2D contact (dx, dy) become the reference's random direction.
-template <int D>
+template <int D, int LC = LC_ANY>
 __device__ __forceinline__ bool pair_term32(const StepParams& P, const Dev& S, const Law32& L, int k, int j, int pidx,
                                             float sp_, float sq_, float r2, uint32_t ts, float& dx, float& dy,
                                             int& nguard, float& f0, bool& contact, float& temp) {
   const float ssum = sq_ + sp_;
+  const int law = LC == LC_ANY ? L.law : LC;
   contact = false;
   temp = 0.0f;
-  if (L.law == CIL_LAW_LJ) {  // LJParams::elemForce (MD.cuh:89-110)
+  if (law == CIL_LAW_LJ) {  // LJParams::elemForce (MD.cuh:89-110)
     const float sh = 0.5f * ssum;
     const float sij2 = sh * sh;
     const float rc2 = sij2 * L.rcut2;
     if (r2 > rc2 * 1.00001f) return false;
-    if (r2 > rc2 * 0.99999f) {  // decided exactly in FP64 by k_lj_guard_fix
+    if (r2 > rc2 * 0.99999f) {  // decided exactly in FP64 by guard_force after the pair loop
       if (nguard < GUARD_SLOTS) S.gslot[(size_t)k * GUARD_SLOTS + nguard] = j;
       ++nguard;
       return false;
     }
-    const float ir2 = 1.0f / r2;
+    const float ir2 = rcp32(r2);
     const float s2 = sij2 * ir2;
     const float s6 = s2 * s2 * s2;
     float f = (s6 - 2.0f * s6 * s6) * ir2;
-    if (L.shifted) f -= lj_f0(sij2, rc2) * sqrtf(rc2 * ir2);
+    if (L.shifted) f -= lj_f0(sij2, rc2) * sqrt32(rc2 * ir2);
     f0 = f * L.eps;
-  } else if (L.law == CIL_LAW_SC) {  // SCParams::elemForce (MD.cuh:190-207)
+  } else if (law == CIL_LAW_SC) {  // SCParams::elemForce (MD.cuh:190-207)
     const float sigCore = 0.5f * ssum;
     const float up = sigCore + L.xi, lo = sigCore - L.xi;
     const float rc2 = L.rcut2 * up * up;
     if (!(r2 > L.rcut2 * lo * lo && r2 <= rc2)) return false;
-    const float rr = sqrtf(r2);
+    const float rr = sqrt32(r2);
+    const float irr = rcp32(rr);
     const float y = rr - sigCore;
-    const float ixi3 = 6.0f / (L.xi * L.xi * L.xi);
     if (y <= -L.xi) f0 = 0.0f;
-    else if (y < 0.0f) f0 = -(L.e0 + L.ew) * ((-y) * (y + L.xi) * ixi3) / rr;
-    else if (y < L.xi) f0 = L.ew * (y * (L.xi - y) * ixi3) / rr;
+    else if (y < 0.0f) f0 = -(L.e0 + L.ew) * ((-y) * (y + L.xi) * L.ixi3) * irr;
+    else if (y < L.xi) f0 = L.ew * (y * (L.xi - y) * L.ixi3) * irr;
     else f0 = 0.0f;
-    if (L.shifted) f0 -= sc_f0(sigCore, rc2, L.xi, L.e0, L.ew) * sqrtf(rc2 / r2);
+    if (L.shifted) f0 -= sc_f0(sigCore, rc2, L.xi, L.e0, L.ew) * (sqrt32(rc2) * irr);
   } else {  // contact laws (device semantics MD.cu:317-362), canonical i = lower index
+    // range test first (symmetric in the two discs), then the partner's index
+    const float rr = sqrt32(r2);
+    if (!(rr <= ssum * 0.5f)) return false;
     const int qidx = S.smeta[j] >> 3;
     const bool p_is_i = pidx < qidx;
     const float si = p_is_i ? sp_ : sq_, sj = p_is_i ? sq_ : sp_;
-    const float rr = sqrtf(r2);
-    if (!(rr <= (si + sj) * 0.5f)) return false;
     contact = true;
     if (D == 2 && dx == 0.0f && dy == 0.0f) {
       double rij[2];
@@ -806,6 +819,44 @@ __device__ __forceinline__ bool pair_term32(const StepParams& P, const Dev& S, c
   return true;
 }
 
+// FP32 LJ pairs of sorted particle k that fell inside the 1e-5 cut-off guard band: the
+// exact FP64 cut-off decision (the reference's predicate) for each, and the FP32 force of
+// those inside, summed in gslot order into F.  Called by the FP32 pair kernels for their own
+// particle after its pair loop (positions are stable during the pass), so no separate pass
+// is launched; F's sum is then added to the pair sum, as one FP32 addition per component.
+__device__ inline void guard_force(const StepParams& P, const Dev& S, int k, int n, float F[3]) {
+  const int D = P.box.dim;
+  const int meta = S.smeta[k];
+  const int pidx = meta >> 3, tp = meta & 7;
+  for (int g = 0; g < n && g < GUARD_SLOTS; ++g) {
+    const int j = S.gslot[(size_t)k * GUARD_SLOTS + g];
+    const int mq = S.smeta[j];
+    const int qidx = mq >> 3, tq = mq & 7;
+    const LawEntry& A = P.law[tp][tq];
+    if (!lj_inside_exact(P, S, pidx, qidx, A.rcut2)) continue;
+    double xp[3], xq[3], d64[3];
+    for (int d = 0; d < D; ++d) {
+      xp[d] = S.pos[(size_t)d * S.P + pidx];
+      xq[d] = S.pos[(size_t)d * S.P + qidx];
+    }
+    box_dist(P.box, d64, xp, xq);
+    float dx[3] = {(float)d64[0], (float)d64[1], D == 3 ? (float)d64[2] : 0.0f};
+    const float r2 = fmaf(dx[0], dx[0], fmaf(dx[1], dx[1], dx[2] * dx[2]));
+    const float sh = 0.5f * (((const float4*)S.spos)[k].w + ((const float4*)S.spos)[j].w);
+    const float sij2 = sh * sh;
+    const float rc2 = sij2 * (float)A.rcut2;
+    const float ir2 = 1.0f / r2;
+    const float s2 = sij2 * ir2;
+    const float s6 = s2 * s2 * s2;
+    float f = (s6 - 2.0f * s6 * s6) * ir2;
+    if (P.force_shifted) f -= lj_f0(sij2, rc2) * sqrtf(rc2 * ir2);
+    float f0 = f * (float)A.eps;
+    const int spec_p = tp / P.nE, spec_q = tq / P.nE;
+    const float fs = (f0 < 0.0f || spec_p == spec_q) ? f0 : (float)P.cross_adh[spec_p] * f0;
+    for (int d = 0; d < D; ++d) F[d] = fmaf(fs, dx[d], F[d]);
+  }
+}
+
 template <int D>
 __device__ inline void k_pair32_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
   const StepParams& P = *Pp;
@@ -830,7 +881,7 @@ __device__ inline void k_pair32_body(const StepParams* __restrict__ Pp, const De
   tp = __builtin_amdgcn_readfirstlane(tp);
   const int k = k0 + lane;
   if (k >= k1) return;
-  const uint32_t ts = (uint32_t)*S.ts;
+  const uint32_t ts = (uint32_t)step_ts(S);
   const int nE = P.nE;
   const int cshift = nE - 1;
   const int spec_p = tp / nE;
@@ -922,12 +973,18 @@ __device__ inline void k_pair32_body(const StepParams* __restrict__ Pp, const De
       }
     }
   }
+  if (nguard > 0) {
+    float g[3] = {0.f, 0.f, 0.f};
+    guard_force(P, S, k, nguard, g);
+    Fx += g[0];
+    Fy += g[1];
+    if (D == 3) Fz += g[2];
+  }
   float* fp = (float*)S.fpair;
   fp[k] = Fx;
   fp[(size_t)S.P + k] = Fy;
   if (D == 3) fp[2 * (size_t)S.P + k] = Fz;
   ((float*)S.ppair)[k] = P.pressure ? pr : 0.0f;
-  S.gcount[k] = nguard;
   if (nguard > GUARD_SLOTS) atomicOr(S.err, (unsigned)ERR_GUARD);
 }
 template <int D>
@@ -943,20 +1000,29 @@ __global__ void __launch_bounds__(256) k_pair32(const StepParams* __restrict__ P
 // ======================================================================================
 __device__ inline int list_code(int ox, int oy, int oz) { return ((ox + 4) * 9 + (oy + 4)) * 9 + (oz + 4); }
 
-// Wave -> (element type, sorted range) of the per-type wave layout used by k_pair32.
-__device__ inline bool wave_type_slot(const StepParams& P, const Dev& S, int bid, int* tp_out, int* k_out) {
+// Wave -> (element type, sorted index) of the list kernels.  Workgroups are dispatched to the
+// XCDs round-robin (workgroup b runs on XCD b % 8), so XCD x takes the x-th contiguous eighth
+// of every element type's waves: each XCD walks one compact region of space, whose neighbour
+// positions stay in its own L2, and gets an equal share of each type (types differ in list
+// length).  Within an XCD the waves run type by type.  The grid holds at least
+// ceil((sum over types of ceil(waves_t / 8)) / waves per block) workgroups per XCD; waves
+// beyond an XCD's share return false.
+__device__ inline bool wave_type_slot(const StepParams& P, const Dev& S, int* tp_out, int* k_out) {
   const int lane = threadIdx.x & (WAVE - 1);
-  const int w = __builtin_amdgcn_readfirstlane(bid * (blockDim.x / WAVE) + threadIdx.x / WAVE);
+  const int x = blockIdx.x % NUM_XCD;
+  const int v = __builtin_amdgcn_readfirstlane((int)(blockIdx.x / NUM_XCD) * (int)(blockDim.x / WAVE) +
+                                               (int)(threadIdx.x / WAVE));
   int tp = -1, k0 = 0, k1 = 0, acc = 0;
   for (int t = 0; t < P.ntypes; ++t) {
     const int T0 = S.start[t * P.nbins], T1 = S.start[(t + 1) * P.nbins];
     const int nw = (T1 - T0 + WAVE - 1) / WAVE;
-    if (tp < 0 && w < acc + nw) {
+    const int lo = (int)(((long)x * nw) / NUM_XCD), hi = (int)(((long)(x + 1) * nw) / NUM_XCD);
+    if (tp < 0 && v < acc + (hi - lo)) {
       tp = t;
-      k0 = T0 + (w - acc) * WAVE;
+      k0 = T0 + (lo + v - acc) * WAVE;
       k1 = T1;
     }
-    acc += nw;
+    acc += hi - lo;
   }
   if (tp < 0) return false;
   *tp_out = __builtin_amdgcn_readfirstlane(tp);
@@ -967,11 +1033,11 @@ __device__ inline bool wave_type_slot(const StepParams& P, const Dev& S, int bid
 // List build (only when the rebuild flag is set): every pair within its own cut-off plus the
 // skin, found with the culled stencil walk of k_pair32; records the build positions.
 template <int D>
-__device__ inline void k_build_list_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
+__device__ inline void k_build_list_body(const StepParams* __restrict__ Pp, const Dev& S) {
   const StepParams& P = *Pp;
   if (*S.rebuild == 0u) return;
   int tp, k;
-  if (!wave_type_slot(P, S, bid, &tp, &k)) return;
+  if (!wave_type_slot(P, S, &tp, &k)) return;
   const float4* __restrict__ spos = (const float4*)S.spos;
   const int* __restrict__ start = S.start;
   const int nE = P.nE, cshift = nE - 1, spec_p = tp / nE;
@@ -1062,133 +1128,141 @@ __device__ inline void k_build_list_body(const StepParams* __restrict__ Pp, cons
 }
 template <int D>
 __global__ void __launch_bounds__(256) k_build_list(const StepParams* __restrict__ Pp, Dev S) {
-  k_build_list_body<D>(Pp, S, xcd_block_chunked(blockIdx.x, gridDim.x));
+  k_build_list_body<D>(Pp, S);
+}
+
+// Bin-offset table of the list kernels, in LDS: entry code -> ((float)ox * wx, (float)oy * wy,
+// (float)oz * wz), the products k_pair32 forms for the same bin offsets.  Filled by the whole
+// block before any wave leaves.
+constexpr int LIST_CODES = 729;
+constexpr int LIST_BLOCK = 256;
+__device__ inline void list_offsets_fill(const StepParams& P, float4* tab) {
+  const float wx = (float)P.w[0], wy = (float)P.w[1], wz = (float)P.w[2];
+  for (int c = threadIdx.x; c < LIST_CODES; c += blockDim.x) {
+    const int ox = c / 81 - 4, oy = (c / 9) % 9 - 4, oz = c % 9 - 4;
+    tab[c] = make_float4((float)ox * wx, (float)oy * wy, (float)oz * wz, 0.0f);
+  }
+  __syncthreads();
+}
+
+// One source-type segment of a particle's list: n entries from byte offset `off` of the list
+// (rows `stride` bytes apart), in list order (the summation order is the list order).  LC is
+// the segment's law class, MI whether any dimension uses the single-bin minimum image.
+template <int D, int LC, bool MI>
+__device__ __forceinline__ void list_segment(const StepParams& P, const Dev& S, const Law32& Lw, const float4* tab,
+                                             const float4& me, int k, int pidx, uint32_t ts, float cadh, int n,
+                                             unsigned& off, unsigned stride, float& Fx, float& Fy, float& Fz,
+                                             float& pr, int& nguard) {
+  const float4* __restrict__ spos = (const float4*)S.spos;
+  const char* __restrict__ nlb = (const char*)S.nl;
+  auto term = [&](unsigned e, const float4& r, const float4& o) {
+    float dx = r.x - (me.x - o.x);
+    float dy = r.y - (me.y - o.y);
+    float dz = D == 3 ? r.z - (me.z - o.z) : 0.0f;
+    if (MI) {
+      if (P.reach[0] == 0 && P.box.pbc[0]) dx -= (float)P.box.L[0] * rintf(dx * (float)P.box.invL[0]);
+      if (P.reach[1] == 0 && P.box.pbc[1]) dy -= (float)P.box.L[1] * rintf(dy * (float)P.box.invL[1]);
+      if (D == 3 && P.reach[2] == 0 && P.box.pbc[2]) dz -= (float)P.box.L[2] * rintf(dz * (float)P.box.invL[2]);
+    }
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    float f0, temp;
+    bool contact;
+    if (!pair_term32<D, LC>(P, S, Lw, k, (int)(e >> 10), pidx, me.w, r.w, r2, ts, dx, dy, nguard, f0, contact, temp))
+      return;
+    const float fs = f0 < 0.0f ? f0 : cadh * f0;
+    Fx = fmaf(fs, dx, Fx);
+    Fy = fmaf(fs, dy, Fy);
+    if (D == 3) Fz = fmaf(fs, dz, Fz);
+    if (LC != CIL_LAW_LJ && LC != CIL_LAW_SC && contact && temp != 0.0f) pr -= fs / temp;
+  };
+  int i = 0;
+  // four entries per round: their list words and positions are loaded before any is used,
+  // so four independent gathers are in flight per lane
+  for (; i + 4 <= n; i += 4) {
+    unsigned e[4];
+    float4 r[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) e[u] = *(const unsigned*)(nlb + (off + (unsigned)u * stride));
+    off += 4u * stride;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) r[u] = spos[e[u] >> 10];
+    float4 o[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) o[u] = tab[e[u] & 1023u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) term(e[u], r[u], o[u]);
+  }
+  for (; i < n; ++i) {
+    const unsigned e = *(const unsigned*)(nlb + off);
+    off += stride;
+    term(e, spos[e >> 10], tab[e & 1023u]);
+  }
 }
 
 // Pair forces over the neighbour list (every step).  Same law code, exclusions, guard band
-// and per-type wave layout as k_pair32; clears the rebuild flag for the next step.
+// and per-type wave layout as k_pair32; clears the rebuild flag for the next step.  List rows
+// are addressed with 32-bit byte offsets (list_max * P * 4 < 2^32, checked by the host).
 template <int D>
-__device__ inline void k_pair_list_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
+__device__ inline void k_pair_list_body(const StepParams* __restrict__ Pp, const Dev& S) {
   const StepParams& P = *Pp;
-  if (bid == 0 && threadIdx.x == 0) *S.rebuild = 0u;
+  __shared__ float4 tab[LIST_CODES];
+  list_offsets_fill(P, tab);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *S.rebuild = 0u;
   int tp, k;
-  if (!wave_type_slot(P, S, bid, &tp, &k)) return;
+  if (!wave_type_slot(P, S, &tp, &k)) return;
   const float4* __restrict__ spos = (const float4*)S.spos;
-  const uint32_t ts = (uint32_t)*S.ts;
+  const uint32_t ts = (uint32_t)step_ts(S);
   const int nE = P.nE, spec_p = tp / nE;
   const float ca_p = (float)P.cross_adh[spec_p];
   const int pidx = S.smeta[k] >> 3;
   const float4 me = spos[k];
-  const float wx = (float)P.w[0], wy = (float)P.w[1], wz = (float)P.w[2];
-  const bool mix = P.reach[0] == 0 && P.box.pbc[0], miy = P.reach[1] == 0 && P.box.pbc[1];
-  const bool miz = D == 3 && P.reach[2] == 0 && P.box.pbc[2];
+  const bool mi = (P.reach[0] == 0 && P.box.pbc[0]) || (P.reach[1] == 0 && P.box.pbc[1]) ||
+                  (D == 3 && P.reach[2] == 0 && P.box.pbc[2]);
   const size_t NP = (size_t)S.P;
+  const unsigned stride = 4u * (unsigned)S.P;
+  unsigned off = 4u * (unsigned)k;
   float Fx = 0.f, Fy = 0.f, Fz = 0.f, pr = 0.f;
-  int nguard = 0, m = 0;
+  int nguard = 0;
   for (int tq = 0; tq < P.ntypes; ++tq) {
     const int n = S.nlt[(size_t)tq * NP + k];
     if (n == 0) continue;
     const Law32 Lw = law32(P, tp, tq);
-    const float rca = (float)P.rca_tt[tp][tq], rcb = (float)P.rcb_tt[tp][tq];
     const float cadh = (tq / nE == spec_p) ? 1.0f : ca_p;  // scale for f0 >= 0 (MD.cu:345-360)
-    // one list entry, in list order (the summation order is the list order)
-    auto term = [&](unsigned e, const float4& r) {
-      const int j = (int)(e >> 10), code = (int)(e & 1023u);
-      float dx = r.x - (me.x - (float)(code / 81 - 4) * wx);
-      float dy = r.y - (me.y - (float)((code / 9) % 9 - 4) * wy);
-      float dz = D == 3 ? r.z - (me.z - (float)(code % 9 - 4) * wz) : 0.0f;
-      if (mix) dx -= (float)P.box.L[0] * rintf(dx * (float)P.box.invL[0]);
-      if (miy) dy -= (float)P.box.L[1] * rintf(dy * (float)P.box.invL[1]);
-      if (miz) dz -= (float)P.box.L[2] * rintf(dz * (float)P.box.invL[2]);
-      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-      const float rc = fmaf(rca, r.w + me.w, rcb);
-      if (r2 > rc * rc * 1.0001f) return;
-      float f0, temp;
-      bool contact;
-      if (!pair_term32<D>(P, S, Lw, k, j, pidx, me.w, r.w, r2, ts, dx, dy, nguard, f0, contact, temp)) return;
-      const float fs = f0 < 0.0f ? f0 : cadh * f0;
-      Fx = fmaf(fs, dx, Fx);
-      Fy = fmaf(fs, dy, Fy);
-      if (D == 3) Fz = fmaf(fs, dz, Fz);
-      if (contact && temp != 0.0f) pr -= fs / temp;
-    };
-    const int m1 = m + n;
-    // four entries per round: their list words and positions are loaded before any is used,
-    // so four independent gathers are in flight per lane
-    for (; m + 4 <= m1; m += 4) {
-      unsigned e[4];
-      float4 r[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) e[u] = (unsigned)S.nl[(size_t)(m + u) * NP + k];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) r[u] = spos[e[u] >> 10];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) term(e[u], r[u]);
+#define CIL_SEG(LC, MIV) \
+  list_segment<D, LC, MIV>(P, S, Lw, tab, me, k, pidx, ts, cadh, n, off, stride, Fx, Fy, Fz, pr, nguard)
+    if (Lw.law == CIL_LAW_LJ) {
+      if (mi) CIL_SEG(CIL_LAW_LJ, true); else CIL_SEG(CIL_LAW_LJ, false);
+    } else if (Lw.law == CIL_LAW_SC) {
+      if (mi) CIL_SEG(CIL_LAW_SC, true); else CIL_SEG(CIL_LAW_SC, false);
+    } else {
+      if (mi) CIL_SEG(LC_CONTACT, true); else CIL_SEG(LC_CONTACT, false);
     }
-    for (; m < m1; ++m) {
-      const unsigned e = (unsigned)S.nl[(size_t)m * NP + k];
-      term(e, spos[e >> 10]);
-    }
+#undef CIL_SEG
+  }
+  if (nguard > 0) {
+    float g[3] = {0.f, 0.f, 0.f};
+    guard_force(P, S, k, nguard, g);
+    Fx += g[0];
+    Fy += g[1];
+    if (D == 3) Fz += g[2];
   }
   float* fp = (float*)S.fpair;
   fp[k] = Fx;
   fp[NP + k] = Fy;
   if (D == 3) fp[2 * NP + k] = Fz;
   ((float*)S.ppair)[k] = P.pressure ? pr : 0.0f;
-  S.gcount[k] = nguard;
   if (nguard > GUARD_SLOTS) atomicOr(S.err, (unsigned)ERR_GUARD);
 }
 template <int D>
 __global__ void __launch_bounds__(256) k_pair_list(const StepParams* __restrict__ Pp, Dev S) {
-  k_pair_list_body<D>(Pp, S, xcd_block_chunked(blockIdx.x, gridDim.x));
+  k_pair_list_body<D>(Pp, S);
 }
 
 __global__ void k_set_flag(unsigned* flag, unsigned v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *flag = v;
 }
 
-__device__ inline void k_lj_guard_fix_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
-  const StepParams& P = *Pp;
-  const int total = S.start[P.nbuckets];
-  const int k = bid * blockDim.x + threadIdx.x;
-  if (k >= total) return;
-  const int n = S.gcount[k];
-  if (n == 0) return;
-  const int D = P.box.dim;
-  const int meta = S.smeta[k];
-  const int pidx = meta >> 3, tp = meta & 7;
-  float* fp = (float*)S.fpair;
-  float F[3] = {0.f, 0.f, 0.f};
-  for (int g = 0; g < n && g < GUARD_SLOTS; ++g) {
-    const int j = S.gslot[(size_t)k * GUARD_SLOTS + g];
-    const int mq = S.smeta[j];
-    const int qidx = mq >> 3, tq = mq & 7;
-    const LawEntry& A = P.law[tp][tq];
-    if (!lj_inside_exact(P, S, pidx, qidx, A.rcut2)) continue;
-    double xp[3], xq[3], d64[3];
-    for (int d = 0; d < D; ++d) {
-      xp[d] = S.pos[(size_t)d * S.P + pidx];
-      xq[d] = S.pos[(size_t)d * S.P + qidx];
-    }
-    box_dist(P.box, d64, xp, xq);
-    float dx[3] = {(float)d64[0], (float)d64[1], D == 3 ? (float)d64[2] : 0.0f};
-    const float r2 = fmaf(dx[0], dx[0], fmaf(dx[1], dx[1], dx[2] * dx[2]));
-    const float sh = 0.5f * (((const float4*)S.spos)[k].w + ((const float4*)S.spos)[j].w);
-    const float sij2 = sh * sh;
-    const float rc2 = sij2 * (float)A.rcut2;
-    const float ir2 = 1.0f / r2;
-    const float s2 = sij2 * ir2;
-    const float s6 = s2 * s2 * s2;
-    float f = (s6 - 2.0f * s6 * s6) * ir2;
-    if (P.force_shifted) f -= lj_f0(sij2, rc2) * sqrtf(rc2 * ir2);
-    float f0 = f * (float)A.eps;
-    const int spec_p = tp / P.nE, spec_q = tq / P.nE;
-    const float fs = (f0 < 0.0f || spec_p == spec_q) ? f0 : (float)P.cross_adh[spec_p] * f0;
-    for (int d = 0; d < D; ++d) F[d] = fmaf(fs, dx[d], F[d]);
-  }
-  for (int d = 0; d < D; ++d) fp[(size_t)d * S.P + k] += F[d];
-}
-__global__ void k_lj_guard_fix(const StepParams* __restrict__ Pp, Dev S) { k_lj_guard_fix_body(Pp, S, (int)blockIdx.x); }
+
 
 // ======================================================================================
 // Noise draws (updateWienerProcess, CIL.cu:793-818), reference noise modes only.
@@ -1200,7 +1274,7 @@ __device__ inline void k_noise_body(const StepParams* __restrict__ Pp, const Dev
   int s, c;
   if (!cell_active(P, S.ncells, cslot, &s, &c)) return;
   const int D = P.box.dim;
-  const int ts = *S.ts;
+  const int ts = step_ts(S);
   const cil_species& sp = P.sp[s];
   const double* blk = nullptr;
   long off_passive = 0, off_active = 0;
@@ -1281,7 +1355,7 @@ __device__ inline void cell_total_force(const StepParams& P, const Dev& S, int s
     *pc = S.pressure[cslot];
     return;
   }
-  intra_cell<DIM, NE>(P, S, s, c, cslot, *S.ts, pos, vel, sig, F, pos1_changed);
+  intra_cell<DIM, NE>(P, S, s, c, cslot, step_ts(S), pos, vel, sig, F, pos1_changed);
   const T* fp = (const T*)S.fpair;
   double pr = 0.0;
   for (int e = 0; e < nE; ++e) {
@@ -1308,20 +1382,23 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
   }
   constexpr int D = DIM, nE = NE;
   const cil_species& sp = P.sp[s];
+  // the intra force reads the old velocity only for motility (one disc) or quadratic drag
+  // (two discs); otherwise it is overwritten without being read
+  const bool need_vel = nE == 1 ? sp.motility[0] != 0.0 : sp.quad_drag != 0;
   double pos[2][3], vel[2][3], sig[2], F[2][3];
   for (int e = 0; e < nE; ++e) {
     size_t p = (size_t)cslot * nE + e;
     sig[e] = S.sigma[p];
     for (int d = 0; d < D; ++d) {
       pos[e][d] = S.pos[(size_t)d * S.P + p];
-      vel[e][d] = S.vel[(size_t)d * S.P + p];
+      vel[e][d] = need_vel ? S.vel[(size_t)d * S.P + p] : 0.0;
     }
   }
   double pc;
   bool pos1_changed;
   cell_total_force<T, DIM, NE>(P, S, s, c, cslot, materialized != 0, pos, vel, sig, F, &pc, &pos1_changed);
   if (!materialized && P.pressure) S.pressure[cslot] = pc;
-  const int tsu = *S.ts;  // step index before the increment (FDT noise key)
+  const int tsu = step_ts(S);  // step index before the increment (FDT noise key)
   double area = sigma2area(sig[0]);
   double pND = pc / sp.cycle_p0;
   for (int e = 0; e < nE; ++e) {

@@ -37,6 +37,7 @@ struct Dev {
   double* awiener;  // [C][2]
   int* ncells;      // [CIL_MAX_SPECIES]
   int* ts;          // [1] step index of the last force evaluation
+  int ts_off;       // added to *ts by every reader: step offset inside a chunk of launched steps
   unsigned* err;    // [1]
   int* key;
   int* rank;
@@ -48,8 +49,8 @@ struct Dev {
   void* spos;
   int* smeta;
   int* sbin;        // spatial bin of each sorted particle
-  int* gslot;       // [P][4] FP32 LJ pairs inside the cut-off guard band (sorted source index)
-  int* gcount;      // [P] number of guard-band pairs per sorted particle
+  int* gslot;       // [P][4] FP32 LJ pairs inside the cut-off guard band (sorted source index),
+                    // resolved in FP64 by the pair kernel itself after its pair loop
   void* fpair;
   void* ppair;
   const double* ext;
@@ -71,6 +72,10 @@ struct Dev {
 // ---------------------------------------------------------------------------------------
 // Small helpers
 // ---------------------------------------------------------------------------------------
+// Step index seen by a kernel: the device counter plus the launch's offset within its chunk
+// (a chunk of steps advances the counter once, at its end).
+__device__ inline int step_ts(const Dev& S) { return *S.ts + S.ts_off; }
+
 __device__ inline bool cell_active(const StepParams& P, const int* ncells, int cslot, int* s_out, int* c_out) {
   int s = species_of_cell(P, cslot);
   int c = cslot - P.base[s];

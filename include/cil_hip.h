@@ -255,7 +255,8 @@ int cil_hip_autotune(cil_ctx* ctx, int reps, int* best_bins, int* best_block, in
  * event records per step. */
 int cil_hip_set_timing(cil_ctx* ctx, int on);
 
-/* Neighbour-list mode (project addition; FP32 force precision, fewer than 2^22 particle slots).
+/* Neighbour-list mode (project addition; FP32 force precision, fewer than 2^22 particle slots,
+ * max_neighbours * particle slots * 4 bytes of list below 4 GiB, 8 <= max_neighbours <= 4096).
  * Pairs within their cut-off + skin are listed at a build; each step evaluates only the list
  * and re-sorts / rebuilds when a particle has moved more than skin / 2 since the build, when
  * cells divide or are removed, or after an upload.  The pair set equals the stencil kernel's
