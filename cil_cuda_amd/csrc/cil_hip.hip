@@ -37,7 +37,16 @@ using namespace cil;
 // fetched into one L2 instead of eight.  A bijection of [0, nblk).
 // Neighbour-list mode gates the sort kernels on the device rebuild flag (S.rebuild is null
 // outside list mode): steps that keep the list skip the whole re-sort with no host round trip.
-__device__ inline bool sort_skipped(const Dev& S) { return S.rebuild != nullptr && *S.rebuild == 0u; }
+__device__ inline bool sort_skipped(const Dev& S) { return S.rebuild != nullptr && *rebuild_flag(S, step_ts(S)) == 0u; }
+
+// Gate of the bucket scan: closed when list mode skips this pass's sort (same test as
+// sort_skipped, for kernels that do not take the Dev).
+struct SortGate {
+  const unsigned* flags;  // null: always open
+  const int* ts;
+  int off;
+};
+__device__ inline bool gate_closed(const SortGate& g) { return g.flags && g.flags[(*g.ts + g.off) & 1] == 0u; }
 
 #define NUM_XCD 8
 __device__ inline int xcd_block(int b, int nblk) {
@@ -87,7 +96,7 @@ __device__ inline void k_keys_body(const StepParams* __restrict__ Pp, const Dev&
 __global__ void k_keys(const StepParams* __restrict__ Pp, Dev S) { k_keys_body(Pp, S, (int)blockIdx.x); }
 
 // ======================================================================================
-// Exclusive prefix sum (3 small kernels): 4096 items per block of 256 threads.
+// Exclusive prefix sum (2 small kernels): 4096 items per block of 256 threads.
 // ======================================================================================
 #define SCAN_ITEMS 16
 #define SCAN_BLOCK 256
@@ -139,36 +148,41 @@ __device__ inline void k_scan_local_body(const int* __restrict__ in, int* __rest
   if (threadIdx.x == 0) bsum[bid] = total;
 }
 __global__ void k_scan_local(const int* __restrict__ in, int* __restrict__ out, int* __restrict__ bsum, int n,
-                             const unsigned* gate) {
-  if (gate && *gate == 0u) return;
+                             SortGate gate) {
+  if (gate_closed(gate)) return;
   k_scan_local_body(in, out, bsum, n, (int)blockIdx.x);
 }
 
-__device__ inline void k_scan_top_body(int* __restrict__ bsum, int nb, int* __restrict__ total_out, int bid) {
-  __shared__ int lds[WAVE + 1];
-  int carry = 0;
-  for (int base = 0; base < nb; base += blockDim.x) {
-    int i = base + threadIdx.x;
-    int v = i < nb ? bsum[i] : 0;
-    int total;
-    int ex = block_exclusive_scan(v, lds, &total);
-    if (i < nb) bsum[i] = ex + carry;
-    carry += total;
-  }
-  if (threadIdx.x == 0) *total_out = carry;
-}
-__global__ void k_scan_top(int* __restrict__ bsum, int nb, int* __restrict__ total_out, const unsigned* gate) {
-  if (gate && *gate == 0u) return;
-  k_scan_top_body(bsum, nb, total_out, (int)blockIdx.x);
+// Sum of v over the workgroup (every thread gets it).
+__device__ inline int block_sum(int v, int* lds) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  if ((threadIdx.x & (WAVE - 1)) == 0) lds[threadIdx.x / WAVE] = v;
+  __syncthreads();
+  int s = 0;
+  for (int k = 0; k < (int)(blockDim.x / WAVE); ++k) s += lds[k];
+  return s;
 }
 
-__device__ inline void k_scan_add_body(int* __restrict__ out, const int* __restrict__ bsum, int n, int bid) {
-  int i = bid * blockDim.x + threadIdx.x;
-  if (i < n) out[i] += bsum[i / SCAN_TILE];
+// Second pass: each 256-item block lies inside one scan tile t and adds the sum of the tile
+// totals before t (at most a few hundred, summed by the block itself); the block holding the
+// last item writes the grand total.  With n == 0 block 0 writes the total (0).
+__device__ inline void k_scan_add_body(int* __restrict__ out, const int* __restrict__ bsum, int n,
+                                       int* __restrict__ total_out, int bid) {
+  __shared__ int lds[WAVE];
+  const int i0 = bid * (int)blockDim.x;
+  const int t = i0 / SCAN_TILE;
+  int acc = 0;
+  for (int q = (int)threadIdx.x; q < t; q += (int)blockDim.x) acc += bsum[q];
+  acc = block_sum(acc, lds);
+  const int i = i0 + (int)threadIdx.x;
+  if (i < n) out[i] += acc;
+  if (total_out && (n > 0 ? i == n - 1 : (bid == 0 && threadIdx.x == 0))) *total_out = acc + bsum[t];
 }
-__global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ bsum, int n, const unsigned* gate) {
-  if (gate && *gate == 0u) return;
-  k_scan_add_body(out, bsum, n, (int)blockIdx.x);
+__global__ void __launch_bounds__(256) k_scan_add(int* __restrict__ out, const int* __restrict__ bsum, int n,
+                                                  int* __restrict__ total_out, SortGate gate) {
+  if (gate_closed(gate)) return;
+  k_scan_add_body(out, bsum, n, total_out, (int)blockIdx.x);
 }
 
 // ======================================================================================
@@ -193,6 +207,7 @@ __device__ inline void wave_lds_sync() {
 
 #define BINSORT_MAX 1024
 #define BINSORT_WAVES 4
+#define BINSORT_GRID 2048  // workgroups of the single-colony bin sort (8 per CU)
 
 template <typename T>
 __device__ inline void write_sorted(const StepParams& P, const Dev& S, int b, int k, int p);
@@ -227,19 +242,13 @@ __device__ inline void write_sorted<double>(const StepParams& P, const Dev& S, i
   ((double4*)S.spos)[k] = r;
 }
 
+// One bucket u (type, spatial bin), sorted by one wave in registers or in its LDS region L.
 template <typename T>
-__device__ inline void k_binsort_body(const StepParams* __restrict__ Pp, const Dev& S, int bid) {
-  const StepParams& P = *Pp;
-  if (sort_skipped(S)) return;
-  __shared__ int buf[BINSORT_WAVES][BINSORT_MAX];
-  int wid = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
-  int u = bid * BINSORT_WAVES + wid;  // bucket (type, spatial bin)
-  if (u >= P.nbuckets) return;
+__device__ inline void binsort_bucket(const StepParams& P, const Dev& S, int u, int lane, int* L) {
   int s0 = S.start[u], n = S.start[u + 1] - s0;
   if (lane == 0) S.count[u] = 0;  // reset for the next step's counting pass
   if (n == 0) return;
   const int b = u % P.nbins;
-  int* L = buf[wid];
   if (n <= WAVE) {
     // register bitonic sort across the wave
     int v = lane < n ? S.idx[s0 + lane] : 0x7fffffff;
@@ -276,6 +285,7 @@ __device__ inline void k_binsort_body(const StepParams* __restrict__ Pp, const D
   // (no workgroup barrier: the other waves of the block take other paths).
   int np2 = 1;
   while (np2 < n) np2 <<= 1;
+  wave_lds_sync();  // the region's previous bucket is fully read
   for (int i = lane; i < np2; i += WAVE) L[i] = i < n ? S.idx[s0 + i] : 0x7fffffff;
   wave_lds_sync();
   for (int k = 2; k <= np2; k <<= 1) {
@@ -307,8 +317,26 @@ __device__ inline void k_binsort_body(const StepParams* __restrict__ Pp, const D
     write_sorted<T>(P, S, b, s0 + i, p);
   }
 }
+// Workgroup bid of nblk takes a contiguous run of bucket groups (BINSORT_WAVES buckets each,
+// one per wave), so a small grid covers every bucket and a skipped sort costs one short launch.
 template <typename T>
-__global__ void __launch_bounds__(WAVE* BINSORT_WAVES) k_binsort(const StepParams* __restrict__ Pp, Dev S) { k_binsort_body<T>(Pp, S, xcd_block(blockIdx.x, gridDim.x)); }
+__device__ inline void k_binsort_body(const StepParams* __restrict__ Pp, const Dev& S, int bid, int nblk) {
+  const StepParams& P = *Pp;
+  if (sort_skipped(S)) return;
+  __shared__ int buf[BINSORT_WAVES][BINSORT_MAX];
+  const int wid = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+  const int groups = (P.nbuckets + BINSORT_WAVES - 1) / BINSORT_WAVES;
+  const int per = (groups + nblk - 1) / nblk;
+  const int g1 = (bid + 1) * per < groups ? (bid + 1) * per : groups;
+  for (int g = bid * per; g < g1; ++g) {
+    const int u = g * BINSORT_WAVES + wid;
+    if (u < P.nbuckets) binsort_bucket<T>(P, S, u, lane, buf[wid]);
+  }
+}
+template <typename T>
+__global__ void __launch_bounds__(WAVE* BINSORT_WAVES) k_binsort(const StepParams* __restrict__ Pp, Dev S) {
+  k_binsort_body<T>(Pp, S, xcd_block(blockIdx.x, gridDim.x), gridDim.x);
+}
 
 // ======================================================================================
 // Pair forces.  Thread per sorted particle: a wavefront holds 64 consecutive particles of
@@ -988,7 +1016,8 @@ __device__ inline void k_pair32_body(const StepParams* __restrict__ Pp, const De
   if (nguard > GUARD_SLOTS) atomicOr(S.err, (unsigned)ERR_GUARD);
 }
 template <int D>
-__global__ void __launch_bounds__(256) k_pair32(const StepParams* __restrict__ Pp, Dev S) { k_pair32_body<D>(Pp, S, xcd_block_chunked(blockIdx.x, gridDim.x)); }
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+k_pair32(const StepParams* __restrict__ Pp, Dev S) { k_pair32_body<D>(Pp, S, xcd_block_chunked(blockIdx.x, gridDim.x)); }
 
 // LJ pairs whose FP32 distance fell within 1e-5 of the cut-off: exact FP64 decision
 // (LJParams::elemForce predicate r^2 <= RCUT^2 sigma_ij^2 on the canonical positions), then
@@ -1030,14 +1059,12 @@ __device__ inline bool wave_type_slot(const StepParams& P, const Dev& S, int* tp
   return k0 + lane < k1;
 }
 
-// List build (only when the rebuild flag is set): every pair within its own cut-off plus the
-// skin, found with the culled stencil walk of k_pair32; records the build positions.
+// List build of sorted particle k (element type tp), run by the list pair kernel when this
+// pass rebuilds: every pair within its own cut-off plus the skin, found with the culled
+// stencil walk of k_pair32; records the build positions.  Only k's own list is written, so
+// the pair loop of the same lane can read it right after.
 template <int D>
-__device__ inline void k_build_list_body(const StepParams* __restrict__ Pp, const Dev& S) {
-  const StepParams& P = *Pp;
-  if (*S.rebuild == 0u) return;
-  int tp, k;
-  if (!wave_type_slot(P, S, &tp, &k)) return;
+__device__ inline void build_list_of(const StepParams& P, const Dev& S, int tp, int k) {
   const float4* __restrict__ spos = (const float4*)S.spos;
   const int* __restrict__ start = S.start;
   const int nE = P.nE, cshift = nE - 1, spec_p = tp / nE;
@@ -1126,10 +1153,6 @@ __device__ inline void k_build_list_body(const StepParams* __restrict__ Pp, cons
   }
   if (over) atomicOr(S.err, (unsigned)ERR_LIST);
 }
-template <int D>
-__global__ void __launch_bounds__(256) k_build_list(const StepParams* __restrict__ Pp, Dev S) {
-  k_build_list_body<D>(Pp, S);
-}
 
 // Bin-offset table of the list kernels, in LDS: entry code -> ((float)ox * wx, (float)oy * wy,
 // (float)oz * wz), the products k_pair32 forms for the same bin offsets.  Filled by the whole
@@ -1199,17 +1222,19 @@ __device__ __forceinline__ void list_segment(const StepParams& P, const Dev& S, 
   }
 }
 
-// Pair forces over the neighbour list (every step).  Same law code, exclusions, guard band
-// and per-type wave layout as k_pair32; clears the rebuild flag for the next step.  List rows
-// are addressed with 32-bit byte offsets (list_max * P * 4 < 2^32, checked by the host).
+// Pair forces over the neighbour list (every step), after rebuilding the particle's list when
+// this pass's rebuild flag is set.  Same law code, exclusions, guard band and per-type wave
+// layout as k_pair32.  List rows are addressed with 32-bit byte offsets
+// (list_max * P * 4 < 2^32, checked by the host).
 template <int D>
 __device__ inline void k_pair_list_body(const StepParams* __restrict__ Pp, const Dev& S) {
   const StepParams& P = *Pp;
   __shared__ float4 tab[LIST_CODES];
   list_offsets_fill(P, tab);
-  if (blockIdx.x == 0 && threadIdx.x == 0) *S.rebuild = 0u;
+  const bool rebuild = *rebuild_flag(S, step_ts(S)) != 0u;
   int tp, k;
   if (!wave_type_slot(P, S, &tp, &k)) return;
+  if (rebuild) build_list_of<D>(P, S, tp, k);
   const float4* __restrict__ spos = (const float4*)S.spos;
   const uint32_t ts = (uint32_t)step_ts(S);
   const int nE = P.nE, spec_p = tp / nE;
@@ -1254,12 +1279,13 @@ __device__ inline void k_pair_list_body(const StepParams* __restrict__ Pp, const
   if (nguard > GUARD_SLOTS) atomicOr(S.err, (unsigned)ERR_GUARD);
 }
 template <int D>
-__global__ void __launch_bounds__(256) k_pair_list(const StepParams* __restrict__ Pp, Dev S) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_pair_list(const StepParams* __restrict__ Pp, Dev S) {
   k_pair_list_body<D>(Pp, S);
 }
 
-__global__ void k_set_flag(unsigned* flag, unsigned v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *flag = v;
+// Both rebuild flags (either step parity) to v.
+__global__ void k_set_flags(unsigned* flag, unsigned v) {
+  if (threadIdx.x < 2 && blockIdx.x == 0) flag[threadIdx.x] = v;
 }
 
 
@@ -1372,6 +1398,9 @@ __device__ inline void cell_total_force(const StepParams& P, const Dev& S, int s
 template <typename T, int DIM, int NE>
 __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const Dev& S, int materialized, int emit_keys, int bid) {
   const StepParams& P = *Pp;
+  // list mode: the flag of the previous force pass (same step parity as this update) has been
+  // used; the skin test below sets the other one, for this step's force pass
+  if (emit_keys == 2 && bid == 0 && threadIdx.x == 0) *rebuild_flag(S, step_ts(S)) = 0u;
   int cslot = bid * blockDim.x + threadIdx.x;
   if (cslot >= S.C) return;
   int s, c;
@@ -1457,8 +1486,9 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
         dr2 = fmaf(dd, dd, dr2);
       }
       if (!finite) atomicOr(S.err, (unsigned)ERR_NONFINITE);
-      if (!(dr2 <= 0.25f * P.skin * P.skin)) atomicOr(S.rebuild, 1u);
-      else if (*S.rebuild == 0u) {
+      unsigned* next = rebuild_flag(S, tsu + 1);  // flag of this step's force pass
+      if (!(dr2 <= 0.25f * P.skin * P.skin)) atomicOr(next, 1u);
+      else if (*next == 0u) {
         const int k = S.inv[p];
         int bc[3];
         bin_coords(P, S.sbin[k], bc);

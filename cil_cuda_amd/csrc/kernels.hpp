@@ -63,7 +63,10 @@ struct Dev {
   int* dblock;
   int* nnew;        // [CIL_MAX_SPECIES]
   // neighbour-list mode
-  unsigned* rebuild;  // [1] nonzero: re-sort and rebuild the list in this force pass
+  unsigned* rebuild;  // [2] by the parity of a force pass's step index: nonzero = that pass
+                      // re-sorts and rebuilds the list (set by the skin test, division and
+                      // host-initiated sorts; the update of step u clears word u & 1, whose
+                      // pass has run)
   int* nl;            // [list_max][P] (sorted index << 9 | bin offset code), per type contiguous
   int* nlt;           // [ntypes][P] entries per source type
   float* xref;        // [3][P] positions at the last build (particle order)
@@ -75,6 +78,8 @@ struct Dev {
 // Step index seen by a kernel: the device counter plus the launch's offset within its chunk
 // (a chunk of steps advances the counter once, at its end).
 __device__ inline int step_ts(const Dev& S) { return *S.ts + S.ts_off; }
+// Rebuild flag of the force pass at step index f (list mode).
+__device__ inline unsigned* rebuild_flag(const Dev& S, int f) { return S.rebuild + (f & 1); }
 
 __device__ inline bool cell_active(const StepParams& P, const int* ncells, int cslot, int* s_out, int* c_out) {
   int s = species_of_cell(P, cslot);

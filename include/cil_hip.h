@@ -252,7 +252,8 @@ int cil_hip_autotune(cil_ctx* ctx, int reps, int* best_bins, int* best_block, in
  * family; any other nonzero value is a mask with bit f+1 set for family f, in the order of
  * the names cil_hip_timings returns (update, division, sort, pair, noise, guard, build, knn,
  * mlp, macro).  Timed steps are launched directly (no graph): each timed family adds two
- * event records per step. */
+ * event records per step.  The FP32 guard-band fix and the neighbour-list build run inside
+ * the pair kernels, so "guard" and "build" stay at zero and "pair" includes them. */
 int cil_hip_set_timing(cil_ctx* ctx, int on);
 
 /* Neighbour-list mode (project addition; FP32 force precision, fewer than 2^22 particle slots,
