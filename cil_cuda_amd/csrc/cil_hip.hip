@@ -1159,6 +1159,12 @@ __device__ inline void build_list_of(const StepParams& P, const Dev& S, int tp, 
 // block before any wave leaves.
 constexpr int LIST_CODES = 729;
 constexpr int LIST_BLOCK = 256;
+#ifndef LIST_UNROLL
+#define LIST_UNROLL 4
+#endif
+#ifndef LIST_WAVES_PER_EU
+#define LIST_WAVES_PER_EU 6
+#endif
 __device__ inline void list_offsets_fill(const StepParams& P, float4* tab) {
   const float wx = (float)P.w[0], wy = (float)P.w[1], wz = (float)P.w[2];
   for (int c = threadIdx.x; c < LIST_CODES; c += blockDim.x) {
@@ -1199,21 +1205,22 @@ __device__ __forceinline__ void list_segment(const StepParams& P, const Dev& S, 
     if (LC != CIL_LAW_LJ && LC != CIL_LAW_SC && contact && temp != 0.0f) pr -= fs / temp;
   };
   int i = 0;
-  // four entries per round: their list words and positions are loaded before any is used,
-  // so four independent gathers are in flight per lane
-  for (; i + 4 <= n; i += 4) {
-    unsigned e[4];
-    float4 r[4];
+  // LIST_UNROLL entries per round: their list words and positions are loaded before any is
+  // used, so that many independent gathers are in flight per lane
+  constexpr int U = LIST_UNROLL;
+  for (; i + U <= n; i += U) {
+    unsigned e[U];
+    float4 r[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) e[u] = *(const unsigned*)(nlb + (off + (unsigned)u * stride));
-    off += 4u * stride;
+    for (int u = 0; u < U; ++u) e[u] = *(const unsigned*)(nlb + (off + (unsigned)u * stride));
+    off += (unsigned)U * stride;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) r[u] = spos[e[u] >> 10];
-    float4 o[4];
+    for (int u = 0; u < U; ++u) r[u] = spos[e[u] >> 10];
+    float4 o[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) o[u] = tab[e[u] & 1023u];
+    for (int u = 0; u < U; ++u) o[u] = tab[e[u] & 1023u];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) term(e[u], r[u], o[u]);
+    for (int u = 0; u < U; ++u) term(e[u], r[u], o[u]);
   }
   for (; i < n; ++i) {
     const unsigned e = *(const unsigned*)(nlb + off);
@@ -1279,7 +1286,8 @@ __device__ inline void k_pair_list_body(const StepParams* __restrict__ Pp, const
   if (nguard > GUARD_SLOTS) atomicOr(S.err, (unsigned)ERR_GUARD);
 }
 template <int D>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_pair_list(const StepParams* __restrict__ Pp, Dev S) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LIST_WAVES_PER_EU)))
+k_pair_list(const StepParams* __restrict__ Pp, Dev S) {
   k_pair_list_body<D>(Pp, S);
 }
 
@@ -1487,8 +1495,11 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
       }
       if (!finite) atomicOr(S.err, (unsigned)ERR_NONFINITE);
       unsigned* next = rebuild_flag(S, tsu + 1);  // flag of this step's force pass
-      if (!(dr2 <= 0.25f * P.skin * P.skin)) atomicOr(next, 1u);
-      else if (*next == 0u) {
+      const bool moved = !(dr2 <= 0.25f * P.skin * P.skin);
+      // one atomic per wave (its first moved lane), none once the flag is up
+      const unsigned long long mv = __ballot(moved);
+      if (mv != 0ull && (int)(threadIdx.x & (WAVE - 1)) == __ffsll(mv) - 1 && *next == 0u) atomicOr(next, 1u);
+      if (!moved && *next == 0u) {
         const int k = S.inv[p];
         int bc[3];
         bin_coords(P, S.sbin[k], bc);
