@@ -209,12 +209,8 @@ __device__ inline void wave_lds_sync() {
 #define BINSORT_WAVES 4
 #define BINSORT_GRID 2048  // workgroups of the single-colony bin sort (8 per CU)
 
-template <typename T>
-__device__ inline void write_sorted(const StepParams& P, const Dev& S, int b, int k, int p);
-
-template <>
-__device__ inline void write_sorted<float>(const StepParams& P, const Dev& S, int b, int k, int p) {
-  int bc[3];
+// Coordinates of linear bin index b (x-major, z fastest in 3D).
+__device__ inline void bin_coords(const StepParams& P, int b, int* bc) {
   if (P.box.dim == 3) {
     bc[2] = b % P.ns[2];
     int t = b / P.ns[2];
@@ -225,6 +221,32 @@ __device__ inline void write_sorted<float>(const StepParams& P, const Dev& S, in
     bc[0] = b / P.ns[1];
     bc[2] = 0;
   }
+}
+
+// Packed bin coordinates of a sorted slot (sbin): 3D x << 21 | y << 10 | z (x, y < 2048,
+// z < 1024), 2D x << 16 | y (x, y < 65536); limits checked when the bins are laid out.
+__device__ inline int pack_bin(const StepParams& P, const int* bc) {
+  return P.box.dim == 3 ? (int)(((unsigned)bc[0] << 21) | ((unsigned)bc[1] << 10) | (unsigned)bc[2])
+                        : (int)(((unsigned)bc[0] << 16) | (unsigned)bc[1]);
+}
+__device__ inline void unpack_bin(const StepParams& P, int s, int* bc) {
+  const unsigned u = (unsigned)s;
+  if (P.box.dim == 3) {
+    bc[0] = (int)(u >> 21);
+    bc[1] = (int)((u >> 10) & 2047u);
+    bc[2] = (int)(u & 1023u);
+  } else {
+    bc[0] = (int)(u >> 16);
+    bc[1] = (int)(u & 0xffffu);
+    bc[2] = 0;
+  }
+}
+
+template <typename T>
+__device__ inline void write_sorted(const StepParams& P, const Dev& S, const int* bc, int k, int p);
+
+template <>
+__device__ inline void write_sorted<float>(const StepParams& P, const Dev& S, const int* bc, int k, int p) {
   float4 r;
   r.x = (float)(S.pos[p] - bc[0] * P.w[0]);
   r.y = (float)(S.pos[(size_t)S.P + p] - bc[1] * P.w[1]);
@@ -233,7 +255,8 @@ __device__ inline void write_sorted<float>(const StepParams& P, const Dev& S, in
   ((float4*)S.spos)[k] = r;
 }
 template <>
-__device__ inline void write_sorted<double>(const StepParams& P, const Dev& S, int b, int k, int p) {
+__device__ inline void write_sorted<double>(const StepParams& P, const Dev& S, const int* bc, int k, int p) {
+  (void)bc;
   double4 r;
   r.x = S.pos[p];
   r.y = S.pos[(size_t)S.P + p];
@@ -242,15 +265,30 @@ __device__ inline void write_sorted<double>(const StepParams& P, const Dev& S, i
   ((double4*)S.spos)[k] = r;
 }
 
-// One bucket u (type, spatial bin), sorted by one wave in registers or in its LDS region L.
+// Outputs of sorted slot k holding particle p of the bin with coordinates bc (packed:
+// pb): index maps, metadata and the sorted coordinates.
 template <typename T>
-__device__ inline void binsort_bucket(const StepParams& P, const Dev& S, int u, int lane, int* L) {
-  int s0 = S.start[u], n = S.start[u + 1] - s0;
-  if (lane == 0) S.count[u] = 0;  // reset for the next step's counting pass
-  if (n == 0) return;
-  const int b = u % P.nbins;
+__device__ inline void emit_sorted(const StepParams& P, const Dev& S, const int* bc, int pb, int k, int p) {
+  S.idx[k] = p;
+  const int cslot = p / P.nE, e = p - cslot * P.nE;
+  const int sp = species_of_cell(P, cslot);
+  const int t = sp * P.nE + e;
+  S.smeta[k] = p * 8 + t;
+  S.sbin[k] = pb;
+  S.inv[p] = k;
+  if (S.sigma[p] > P.sigma_bound[t]) atomicOr(S.err, (unsigned)ERR_CUTOFF);
+  write_sorted<T>(P, S, bc, k, p);
+}
+
+// A bucket of more than BINSORT_SMALL particles, sorted by the whole wave in registers
+// (bitonic across lanes) or in its LDS region L.
+template <typename T>
+__device__ inline void binsort_bucket_wave(const StepParams& P, const Dev& S, int u, int lane, int* L) {
+  const int s0 = S.start[u], n = S.start[u + 1] - s0;
+  int bc[3];
+  bin_coords(P, u % P.nbins, bc);
+  const int pb = pack_bin(P, bc);
   if (n <= WAVE) {
-    // register bitonic sort across the wave
     int v = lane < n ? S.idx[s0 + lane] : 0x7fffffff;
 #pragma unroll
     for (int k = 2; k <= WAVE; k <<= 1) {
@@ -263,18 +301,7 @@ __device__ inline void binsort_bucket(const StepParams& P, const Dev& S, int u, 
         v = (lower == up) ? mn : mx;
       }
     }
-    if (lane < n) {
-      S.idx[s0 + lane] = v;
-      int p = v;
-      int cslot = p / P.nE, e = p - cslot * P.nE;
-      int sp = species_of_cell(P, cslot);
-      int t = sp * P.nE + e;
-      S.smeta[s0 + lane] = p * 8 + t;
-      S.sbin[s0 + lane] = b;
-      S.inv[p] = s0 + lane;
-      if (S.sigma[p] > P.sigma_bound[t]) atomicOr(S.err, (unsigned)ERR_CUTOFF);
-      write_sorted<T>(P, S, b, s0 + lane, p);
-    }
+    if (lane < n) emit_sorted<T>(P, S, bc, pb, s0 + lane, v);
     return;
   }
   if (n > BINSORT_MAX) {
@@ -304,33 +331,63 @@ __device__ inline void binsort_bucket(const StepParams& P, const Dev& S, int u, 
       wave_lds_sync();
     }
   }
-  for (int i = lane; i < n; i += WAVE) {
-    int p = L[i];
-    S.idx[s0 + i] = p;
-    int cslot = p / P.nE, e = p - cslot * P.nE;
-    int sp = species_of_cell(P, cslot);
-    int t = sp * P.nE + e;
-    S.smeta[s0 + i] = p * 8 + t;
-    S.sbin[s0 + i] = b;
-    S.inv[p] = s0 + i;
-    if (S.sigma[p] > P.sigma_bound[t]) atomicOr(S.err, (unsigned)ERR_CUTOFF);
-    write_sorted<T>(P, S, b, s0 + i, p);
+  wave_lds_sync();
+  for (int i = lane; i < n; i += WAVE) emit_sorted<T>(P, S, bc, pb, s0 + i, L[i]);
+}
+
+// Compare-exchange of Batcher's odd-even merge sort network for 8 keys (19 comparators).
+#define BINSORT_SMALL 8
+__device__ inline void sort8(int* v) {
+  constexpr int net[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
+                              {0, 4}, {1, 5}, {2, 6}, {3, 7}, {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+  for (int c = 0; c < 19; ++c) {
+    const int a = v[net[c][0]], d = v[net[c][1]];
+    v[net[c][0]] = a < d ? a : d;
+    v[net[c][1]] = a < d ? d : a;
   }
 }
-// Workgroup bid of nblk takes a contiguous run of bucket groups (BINSORT_WAVES buckets each,
-// one per wave), so a small grid covers every bucket and a skipped sort costs one short launch.
+
+// Wave w of nw takes a contiguous run of 64-bucket groups.  In a group, lane l sorts bucket
+// base + l by itself when it holds at most BINSORT_SMALL particles (the common case: a bin
+// holds one or two); larger buckets are then sorted one at a time by the whole wave.  Every
+// bucket ends in ascending particle index, so the order is the same on every run.
 template <typename T>
 __device__ inline void k_binsort_body(const StepParams* __restrict__ Pp, const Dev& S, int bid, int nblk) {
   const StepParams& P = *Pp;
   if (sort_skipped(S)) return;
   __shared__ int buf[BINSORT_WAVES][BINSORT_MAX];
   const int wid = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
-  const int groups = (P.nbuckets + BINSORT_WAVES - 1) / BINSORT_WAVES;
-  const int per = (groups + nblk - 1) / nblk;
-  const int g1 = (bid + 1) * per < groups ? (bid + 1) * per : groups;
-  for (int g = bid * per; g < g1; ++g) {
-    const int u = g * BINSORT_WAVES + wid;
-    if (u < P.nbuckets) binsort_bucket<T>(P, S, u, lane, buf[wid]);
+  const int groups = (P.nbuckets + WAVE - 1) / WAVE;
+  const int nw = nblk * BINSORT_WAVES, w = bid * BINSORT_WAVES + wid;
+  const int per = (groups + nw - 1) / nw;
+  const int g1 = (w + 1) * per < groups ? (w + 1) * per : groups;
+  for (int g = w * per; g < g1; ++g) {
+    const int u = g * WAVE + lane;
+    int n = 0, s0 = 0;
+    if (u < P.nbuckets) {
+      s0 = S.start[u];
+      n = S.start[u + 1] - s0;
+      S.count[u] = 0;  // reset for the next counting pass
+    }
+    if (n > 0 && n <= BINSORT_SMALL) {
+      int v[BINSORT_SMALL];
+#pragma unroll
+      for (int i = 0; i < BINSORT_SMALL; ++i) v[i] = i < n ? S.idx[s0 + i] : 0x7fffffff;
+      sort8(v);
+      int bc[3];
+      bin_coords(P, u % P.nbins, bc);
+      const int pb = pack_bin(P, bc);
+#pragma unroll
+      for (int i = 0; i < BINSORT_SMALL; ++i)
+        if (i < n) emit_sorted<T>(P, S, bc, pb, s0 + i, v[i]);
+    }
+    unsigned long long big = __ballot(n > BINSORT_SMALL);
+    while (big) {
+      const int l = __ffsll(big) - 1;
+      big &= big - 1;
+      binsort_bucket_wave<T>(P, S, __shfl(u, l, WAVE), lane, buf[wid]);
+    }
   }
 }
 template <typename T>
@@ -346,18 +403,6 @@ __global__ void __launch_bounds__(WAVE* BINSORT_WAVES) k_binsort(const StepParam
 // the periodic shift.  Law parameters of the (target, source) type pair sit in registers;
 // every candidate gets a conservative cut-off test, then the exact reference predicate.
 // ======================================================================================
-__device__ inline void bin_coords(const StepParams& P, int b, int* bc) {
-  if (P.box.dim == 3) {
-    bc[2] = b % P.ns[2];
-    int t = b / P.ns[2];
-    bc[1] = t % P.ns[1];
-    bc[0] = t / P.ns[1];
-  } else {
-    bc[1] = b % P.ns[1];
-    bc[0] = b / P.ns[1];
-    bc[2] = 0;
-  }
-}
 
 // Law of one (target p, source q) element-type pair with both orientations' asymmetric
 // terms: the canonical "i" of a pair is its lower particle index (lower UID).
@@ -631,7 +676,7 @@ __device__ inline void k_pair_body(const StepParams* __restrict__ Pp, const Dev&
   const int cell_p = pidx >> cshift;
   const int spec_p = species_of_cell(P, cell_p);
   int bc[3];
-  bin_coords(P, S.sbin[k], bc);
+  unpack_bin(P, S.sbin[k], bc);
   T xp[4];
   if constexpr (sizeof(T) == 4) {
     float4 r = ((const float4*)S.spos)[k];
@@ -917,7 +962,7 @@ __device__ inline void k_pair32_body(const StepParams* __restrict__ Pp, const De
   const int pidx = smeta[k] >> 3;
   const int cell_p = pidx >> cshift;
   int bc[3];
-  bin_coords(P, S.sbin[k], bc);
+  unpack_bin(P, S.sbin[k], bc);
   const float4 me = spos[k];
   const float wx = (float)P.w[0], wy = (float)P.w[1], wz = (float)P.w[2];
   const bool mix = P.reach[0] == 0 && P.box.pbc[0], miy = P.reach[1] == 0 && P.box.pbc[1];
@@ -1071,7 +1116,7 @@ __device__ inline void build_list_of(const StepParams& P, const Dev& S, int tp, 
   const int pidx = S.smeta[k] >> 3, cell_p = pidx >> cshift;
   for (int d = 0; d < D; ++d) S.xref[(size_t)d * S.P + pidx] = (float)S.pos[(size_t)d * S.P + pidx];
   int bc[3];
-  bin_coords(P, S.sbin[k], bc);
+  unpack_bin(P, S.sbin[k], bc);
   const float4 me = spos[k];
   const float wx = (float)P.w[0], wy = (float)P.w[1], wz = (float)P.w[2];
   const bool mix = P.reach[0] == 0 && P.box.pbc[0], miy = P.reach[1] == 0 && P.box.pbc[1];
@@ -1502,7 +1547,7 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
       if (!moved && *next == 0u) {
         const int k = S.inv[p];
         int bc[3];
-        bin_coords(P, S.sbin[k], bc);
+        unpack_bin(P, S.sbin[k], bc);
         float* sp = (float*)S.spos + 4 * (size_t)k;  // x, y, z of the float4 slot; sigma stays
         for (int d = 0; d < D; ++d) {
           double l = xn[d] - bc[d] * P.w[d];
@@ -1513,8 +1558,11 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
     }
   }
 }
+// Register budget: six waves per SIMD for single-disc cells (no spills); two-disc cells keep
+// the compiler's own allocation.
 template <typename T, int DIM, int NE>
-__global__ void __launch_bounds__(256) k_update(const StepParams* __restrict__ Pp, Dev S, int materialized,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE == 1 ? 6 : 1)))
+k_update(const StepParams* __restrict__ Pp, Dev S, int materialized,
                                                 int emit_keys) {
   k_update_body<T, DIM, NE>(Pp, S, materialized, emit_keys, (int)blockIdx.x);
 }
@@ -1573,7 +1621,7 @@ __global__ void k_pairs_list(const StepParams* __restrict__ Pp, Dev S, int reach
   for (int d = 0; d < D; ++d) xp[d] = S.pos[(size_t)d * S.P + pidx];
   double sp_ = S.sigma[pidx];
   int bc[3];
-  bin_coords(P, S.sbin[k], bc);
+  unpack_bin(P, S.sbin[k], bc);
   int reach[3] = {reach_x, reach_y, D == 3 ? reach_z : 0};
   for (int tq = 0; tq < P.ntypes; ++tq) {
     const LawEntry& E = P.law[tp][tq];

@@ -48,7 +48,7 @@ struct Dev {
   int* inv;
   void* spos;
   int* smeta;
-  int* sbin;        // spatial bin of each sorted particle
+  int* sbin;        // bin of each sorted particle, packed coordinates (pack_bin)
   int* gslot;       // [P][4] FP32 LJ pairs inside the cut-off guard band (sorted source index),
                     // resolved in FP64 by the pair kernel itself after its pair loop
   void* fpair;
@@ -113,15 +113,20 @@ __device__ inline int bucket_key(const StepParams& P, int p, int s, const double
 // passive noise.  Writes the full intra force of each element of the cell into F.
 // `vel` may be modified (singleCellForce's zero-velocity kick), pos0/pos1 too (zero bond).
 // ---------------------------------------------------------------------------------------
+// The axis is a run-time value: the component is picked with selects, not an index, so the
+// caller's position and force arrays stay in registers.
 __device__ inline void flat_wall(const cil_wall& wl, double sigma, const double* r, double* f) {
-  double dr = (wl.positive ? (r[wl.axis] - wl.x0) : (wl.x0 - r[wl.axis])) / sigma;
+  const double ra = wl.axis == 0 ? r[0] : (wl.axis == 1 ? r[1] : r[2]);
+  double dr = (wl.positive ? (ra - wl.x0) : (wl.x0 - ra)) / sigma;
   if (dr <= wl.rcut) {
     dr = dr >= kWallCutoff ? dr : kWallCutoff;
     double ir2 = 1.0 / (dr * dr);
     double f0 = ir2 * ir2 * ir2;
     f0 = wl.epsilon * (2.0 * f0 - 1.0) * f0 * ir2 / sq(sigma);
     f0 = wl.positive ? f0 : -f0;
-    f[wl.axis] += f0;
+    if (wl.axis == 0) f[0] += f0;
+    else if (wl.axis == 1) f[1] += f0;
+    else f[2] += f0;
   }
 }
 
