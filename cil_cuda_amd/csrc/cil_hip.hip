@@ -1492,8 +1492,13 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
     bool fdt = P.noise == CIL_NOISE_FDT && sp.kT > 0.0;
     if (fdt) {
       uint32_t id = (cell_key(s, c) << 1) | (uint32_t)e;
-      philox_normal2(P.seed, id, (uint32_t)tsu, ST_FDT, 0, z);
-      philox_normal2(P.seed, id, (uint32_t)tsu, ST_FDT, 1, z + 2);
+      if (sizeof(T) == 4) {  // FP32 force mode: FP32 Box-Muller on the same draws
+        philox_normal2_f32(P.seed, id, (uint32_t)tsu, ST_FDT, 0, z);
+        if (D == 3) philox_normal2_f32(P.seed, id, (uint32_t)tsu, ST_FDT, 1, z + 2);
+      } else {
+        philox_normal2(P.seed, id, (uint32_t)tsu, ST_FDT, 0, z);
+        if (D == 3) philox_normal2(P.seed, id, (uint32_t)tsu, ST_FDT, 1, z + 2);
+      }
     }
     double amp = fdt ? sqrt(2.0 * sp.kT * temp) : 0.0;
     double xn[3];
