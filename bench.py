@@ -145,6 +145,9 @@ def copy_calibration(nbytes=256 << 20, reps=3):
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
 
 
+LIST_BINS_PER_CUTOFF = 2
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,10 +182,14 @@ def main():
     md = HipMD(p, box, [sp["capacity"] for sp in col.species])
     if args.bins_per_cutoff:
         md.set_tuning(args.bins_per_cutoff, args.block)
+    elif not args.no_list:
+        # list mode: bins sized (cut-off + skin) / 2 measured fastest (1, 2 and 3 bins per
+        # cut-off at 1M particles, skin 0.3); the stencil autotuner's objective does not apply
+        md.set_tuning(LIST_BINS_PER_CUTOFF, 64)
     md.upload(col)
     md.forces(want=False)
     tuned = None
-    if not args.bins_per_cutoff:
+    if not args.bins_per_cutoff and args.no_list:
         tuned = md.autotune(reps=3)
         log(f"autotune: {tuned}")
     if not args.no_list:

@@ -1219,6 +1219,11 @@ __device__ inline void list_offsets_fill(const StepParams& P, float4* tab) {
   __syncthreads();
 }
 
+// A list word (plain cached load: non-temporal loads of the list measured slower).
+__device__ __forceinline__ unsigned list_word(const char* __restrict__ nlb, unsigned off) {
+  return *(const unsigned*)(nlb + off);
+}
+
 // One source-type segment of a particle's list: n entries from byte offset `off` of the list
 // (rows `stride` bytes apart), in list order (the summation order is the list order).  LC is
 // the segment's law class, MI whether any dimension uses the single-bin minimum image.
@@ -1257,7 +1262,7 @@ __device__ __forceinline__ void list_segment(const StepParams& P, const Dev& S, 
     unsigned e[U];
     float4 r[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) e[u] = *(const unsigned*)(nlb + (off + (unsigned)u * stride));
+    for (int u = 0; u < U; ++u) e[u] = list_word(nlb, off + (unsigned)u * stride);
     off += (unsigned)U * stride;
 #pragma unroll
     for (int u = 0; u < U; ++u) r[u] = spos[e[u] >> 10];
@@ -1268,7 +1273,7 @@ __device__ __forceinline__ void list_segment(const StepParams& P, const Dev& S, 
     for (int u = 0; u < U; ++u) term(e[u], r[u], o[u]);
   }
   for (; i < n; ++i) {
-    const unsigned e = *(const unsigned*)(nlb + off);
+    const unsigned e = list_word(nlb, off);
     off += stride;
     term(e, spos[e >> 10], tab[e & 1023u]);
   }
