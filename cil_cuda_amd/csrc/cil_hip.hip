@@ -1219,6 +1219,40 @@ __device__ inline void list_offsets_fill(const StepParams& P, float4* tab) {
   __syncthreads();
 }
 
+// Branch-free LJ and soft-core terms of the list kernel: the same operations and roundings as
+// pair_term32, with out-of-range pairs giving f0 = 0 by selection (their fmaf then leaves the
+// sums unchanged), so every lane runs one instruction stream.  lj_term32 sets *guard for a
+// pair inside the 1e-5 cut-off guard band (f0 = 0; decided in FP64 later).
+__device__ __forceinline__ float lj_term32(const Law32& L, float ssum, float r2, bool* guard) {
+  const float sh = 0.5f * ssum;
+  const float sij2 = sh * sh;
+  const float rc2 = sij2 * L.rcut2;
+  const bool in = r2 <= rc2 * 0.99999f;
+  *guard = !in && r2 <= rc2 * 1.00001f;
+  const float ir2 = rcp32(r2);
+  const float s2 = sij2 * ir2;
+  const float s6 = s2 * s2 * s2;
+  float f = (s6 - 2.0f * s6 * s6) * ir2;
+  if (L.shifted) f -= lj_f0(sij2, rc2) * sqrt32(rc2 * ir2);
+  return in ? f * L.eps : 0.0f;
+}
+__device__ __forceinline__ float sc_term32(const Law32& L, float ssum, float r2) {
+  const float sigCore = 0.5f * ssum;
+  const float up = sigCore + L.xi, lo = sigCore - L.xi;
+  const float rc2 = L.rcut2 * up * up;
+  const bool valid = r2 > L.rcut2 * lo * lo && r2 <= rc2;
+  const float rr = sqrt32(r2);
+  const float irr = rcp32(rr);
+  const float y = rr - sigCore;
+  const bool neg = y < 0.0f;
+  const float a = neg ? -y : y, b = neg ? (y + L.xi) : (L.xi - y);
+  const float coef = neg ? -(L.e0 + L.ew) : L.ew;
+  const float f = coef * (a * b * L.ixi3) * irr;
+  float f0 = (valid && y > -L.xi && y < L.xi) ? f : 0.0f;
+  if (L.shifted) f0 = valid ? f0 - sc_f0(sigCore, rc2, L.xi, L.e0, L.ew) * (sqrt32(rc2) * irr) : 0.0f;
+  return f0;
+}
+
 // A list word (plain cached load: non-temporal loads of the list measured slower).
 __device__ __forceinline__ unsigned list_word(const char* __restrict__ nlb, unsigned off) {
   return *(const unsigned*)(nlb + off);
@@ -1244,15 +1278,33 @@ __device__ __forceinline__ void list_segment(const StepParams& P, const Dev& S, 
       if (D == 3 && P.reach[2] == 0 && P.box.pbc[2]) dz -= (float)P.box.L[2] * rintf(dz * (float)P.box.invL[2]);
     }
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    float f0, temp;
-    bool contact;
-    if (!pair_term32<D, LC>(P, S, Lw, k, (int)(e >> 10), pidx, me.w, r.w, r2, ts, dx, dy, nguard, f0, contact, temp))
+    float f0;
+    if (LC == CIL_LAW_LJ) {
+      bool guard;
+      f0 = lj_term32(Lw, r.w + me.w, r2, &guard);
+      if (guard) {  // rare: recorded for the exact FP64 decision after the loop
+        if (nguard < GUARD_SLOTS) S.gslot[(size_t)k * GUARD_SLOTS + nguard] = (int)(e >> 10);
+        ++nguard;
+      }
+    } else if (LC == CIL_LAW_SC) {
+      f0 = sc_term32(Lw, r.w + me.w, r2);
+    } else {
+      float temp;
+      bool contact;
+      if (!pair_term32<D, LC>(P, S, Lw, k, (int)(e >> 10), pidx, me.w, r.w, r2, ts, dx, dy, nguard, f0, contact,
+                              temp))
+        return;
+      const float fs = f0 < 0.0f ? f0 : cadh * f0;
+      Fx = fmaf(fs, dx, Fx);
+      Fy = fmaf(fs, dy, Fy);
+      if (D == 3) Fz = fmaf(fs, dz, Fz);
+      if (contact && temp != 0.0f) pr -= fs / temp;
       return;
+    }
     const float fs = f0 < 0.0f ? f0 : cadh * f0;
     Fx = fmaf(fs, dx, Fx);
     Fy = fmaf(fs, dy, Fy);
     if (D == 3) Fz = fmaf(fs, dz, Fz);
-    if (LC != CIL_LAW_LJ && LC != CIL_LAW_SC && contact && temp != 0.0f) pr -= fs / temp;
   };
   int i = 0;
   // LIST_UNROLL entries per round: their list words and positions are loaded before any is
