@@ -262,7 +262,11 @@ def main():
         for s_ in range(q.nspecies):
             q.species[s_].kT = 0.005
         m2 = HipMD(q, box, [sp["capacity"] for sp in col.species])
+        if not args.no_list:  # the same plain path as the main leg for the fallback steps
+            m2.set_tuning(args.bins_per_cutoff or LIST_BINS_PER_CUTOFF, args.block or 64)
         m2.upload(col)
+        if not args.no_list:
+            m2.set_neighbour_list(True, args.skin, args.list_max)
         m2.forces(want=False)
         res = surrogate_leg(m2, 0, n, value, args, q, box, col)
         res["kT"] = 0.005

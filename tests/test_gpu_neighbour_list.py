@@ -171,6 +171,21 @@ def test_list_division_and_drain_match_stencil(gpu):
     assert _max_disp_err(out[0], out[1], box) <= 1e-5
 
 
+def test_list_overflow_is_reported(gpu):
+    # 8 entries per particle cannot hold a dense bulk's lists: the build stores at most 8 and
+    # raises the list error, which the step returns as a capacity error with a message
+    p, box, col = _relaxed(4096, 6)
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    md.set_neighbour_list(True, 0.3, 8)
+    with pytest.raises(abi.CilError) as ei:
+        md.forces(want=False)
+        md.step(1, 0)
+    say("list overflow:", ei.value)
+    assert "max_neighbours" in str(ei.value)
+    md.close()
+
+
 def test_list_mode_rejects_fp64(gpu):
     p, box, col = configs.plain_2d_sc(n_cells=50, L=12.0, seed=3)
     md = HipMD(p, box, [sp["capacity"] for sp in col.species])
