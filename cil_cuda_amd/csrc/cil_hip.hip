@@ -1223,19 +1223,24 @@ __device__ inline void list_offsets_fill(const StepParams& P, float4* tab) {
 // pair_term32, with out-of-range pairs giving f0 = 0 by selection (their fmaf then leaves the
 // sums unchanged), so every lane runs one instruction stream.  lj_term32 sets *guard for a
 // pair inside the 1e-5 cut-off guard band (f0 = 0; decided in FP64 later).
+// SH: the law is force-shifted (compile-time, so the loop carries no run-time test).  The
+// guard band is |r2 - rc2| <= 1e-5 rc2; pairs below it interact, pairs above do not.
+template <bool SH>
 __device__ __forceinline__ float lj_term32(const Law32& L, float ssum, float r2, bool* guard) {
   const float sh = 0.5f * ssum;
   const float sij2 = sh * sh;
   const float rc2 = sij2 * L.rcut2;
-  const bool in = r2 <= rc2 * 0.99999f;
-  *guard = !in && r2 <= rc2 * 1.00001f;
+  const float dr = r2 - rc2, tol = rc2 * 1e-5f;
+  const bool in = dr < -tol;
+  *guard = fabsf(dr) <= tol;
   const float ir2 = rcp32(r2);
   const float s2 = sij2 * ir2;
   const float s6 = s2 * s2 * s2;
   float f = (s6 - 2.0f * s6 * s6) * ir2;
-  if (L.shifted) f -= lj_f0(sij2, rc2) * sqrt32(rc2 * ir2);
+  if (SH) f -= lj_f0(sij2, rc2) * sqrt32(rc2 * ir2);
   return in ? f * L.eps : 0.0f;
 }
+template <bool SH>
 __device__ __forceinline__ float sc_term32(const Law32& L, float ssum, float r2) {
   const float sigCore = 0.5f * ssum;
   const float up = sigCore + L.xi, lo = sigCore - L.xi;
@@ -1249,7 +1254,7 @@ __device__ __forceinline__ float sc_term32(const Law32& L, float ssum, float r2)
   const float coef = neg ? -(L.e0 + L.ew) : L.ew;
   const float f = coef * (a * b * L.ixi3) * irr;
   float f0 = (valid && y > -L.xi && y < L.xi) ? f : 0.0f;
-  if (L.shifted) f0 = valid ? f0 - sc_f0(sigCore, rc2, L.xi, L.e0, L.ew) * (sqrt32(rc2) * irr) : 0.0f;
+  if (SH) f0 = valid ? f0 - sc_f0(sigCore, rc2, L.xi, L.e0, L.ew) * (sqrt32(rc2) * irr) : 0.0f;
   return f0;
 }
 
@@ -1261,7 +1266,7 @@ __device__ __forceinline__ unsigned list_word(const char* __restrict__ nlb, unsi
 // One source-type segment of a particle's list: n entries from byte offset `off` of the list
 // (rows `stride` bytes apart), in list order (the summation order is the list order).  LC is
 // the segment's law class, MI whether any dimension uses the single-bin minimum image.
-template <int D, int LC, bool MI>
+template <int D, int LC, bool MI, bool SH>
 __device__ __forceinline__ void list_segment(const StepParams& P, const Dev& S, const Law32& Lw, const float4* tab,
                                              const float4& me, int k, int pidx, uint32_t ts, float cadh, int n,
                                              unsigned& off, unsigned stride, float& Fx, float& Fy, float& Fz,
@@ -1281,13 +1286,13 @@ __device__ __forceinline__ void list_segment(const StepParams& P, const Dev& S, 
     float f0;
     if (LC == CIL_LAW_LJ) {
       bool guard;
-      f0 = lj_term32(Lw, r.w + me.w, r2, &guard);
+      f0 = lj_term32<SH>(Lw, r.w + me.w, r2, &guard);
       if (guard) {  // rare: recorded for the exact FP64 decision after the loop
         if (nguard < GUARD_SLOTS) S.gslot[(size_t)k * GUARD_SLOTS + nguard] = (int)(e >> 10);
         ++nguard;
       }
     } else if (LC == CIL_LAW_SC) {
-      f0 = sc_term32(Lw, r.w + me.w, r2);
+      f0 = sc_term32<SH>(Lw, r.w + me.w, r2);
     } else {
       float temp;
       bool contact;
@@ -1362,14 +1367,22 @@ __device__ inline void k_pair_list_body(const StepParams* __restrict__ Pp, const
     if (n == 0) continue;
     const Law32 Lw = law32(P, tp, tq);
     const float cadh = (tq / nE == spec_p) ? 1.0f : ca_p;  // scale for f0 >= 0 (MD.cu:345-360)
-#define CIL_SEG(LC, MIV) \
-  list_segment<D, LC, MIV>(P, S, Lw, tab, me, k, pidx, ts, cadh, n, off, stride, Fx, Fy, Fz, pr, nguard)
+#define CIL_SEG(LC, MIV, SHV) \
+  list_segment<D, LC, MIV, SHV>(P, S, Lw, tab, me, k, pidx, ts, cadh, n, off, stride, Fx, Fy, Fz, pr, nguard)
     if (Lw.law == CIL_LAW_LJ) {
-      if (mi) CIL_SEG(CIL_LAW_LJ, true); else CIL_SEG(CIL_LAW_LJ, false);
+      if (Lw.shifted) {
+        if (mi) CIL_SEG(CIL_LAW_LJ, true, true); else CIL_SEG(CIL_LAW_LJ, false, true);
+      } else {
+        if (mi) CIL_SEG(CIL_LAW_LJ, true, false); else CIL_SEG(CIL_LAW_LJ, false, false);
+      }
     } else if (Lw.law == CIL_LAW_SC) {
-      if (mi) CIL_SEG(CIL_LAW_SC, true); else CIL_SEG(CIL_LAW_SC, false);
+      if (Lw.shifted) {
+        if (mi) CIL_SEG(CIL_LAW_SC, true, true); else CIL_SEG(CIL_LAW_SC, false, true);
+      } else {
+        if (mi) CIL_SEG(CIL_LAW_SC, true, false); else CIL_SEG(CIL_LAW_SC, false, false);
+      }
     } else {
-      if (mi) CIL_SEG(LC_CONTACT, true); else CIL_SEG(LC_CONTACT, false);
+      if (mi) CIL_SEG(LC_CONTACT, true, false); else CIL_SEG(LC_CONTACT, false, false);
     }
 #undef CIL_SEG
   }

@@ -7,8 +7,9 @@
 // Per-step scratch (rebuilt every step):
 //   key/rank    : int32 [P]  bin index and arrival rank (atomic counting sort)
 //   count/start : int32 [nbins(+1)]
-//   idx         : int32 [P]  sorted slot -> particle, order inside a bin fixed by an
-//                            in-wave bitonic sort so every run is bit-reproducible
+//   idx         : int32 [P]  sorted slot -> particle, ascending particle index inside a bin
+//                            (sorting network per lane, wave bitonic for large bins), so
+//                            every run is bit-reproducible
 //   spos        : float4 (bin-local x,y,z, sigma) in FP32 mode, double4 (global) in FP64
 //   smeta       : int32 [P]  particle*8 + element type
 //   fpair/ppair : [D][P] / [P] pair forces and pressure partials, in sorted order
