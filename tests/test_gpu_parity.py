@@ -62,6 +62,39 @@ def _random_colony(dim, nelem, n, L, periodic, law, seed, nspec=2):
     return p, box, col
 
 
+@pytest.mark.parametrize("mode", ["fp64", "fp32", "list32"])
+@pytest.mark.parametrize("n,L", [(12000, 14.0), (300, 3.4)])
+def test_forces_crowded_buckets(gpu, mode, n, L):
+    """Buckets beyond the bin sort's per-lane network (more than 8 particles: wave-wide sort in
+    registers) and beyond one wave (more than 64: sort in LDS): bins of one cut-off in a dense
+    box, and a box of a single bin.  Forces against the FP64 oracle.  Soft core: these random
+    colonies overlap, and the LJ r^-13 wall would turn FP32 coordinate rounding at near-zero
+    distances into errors beyond the FP32 bar, which is not what this test is about."""
+    p, box, col = _random_colony(3, 1, n, L, 1, abi.LAW_SC, seed=17)
+    p.precision = abi.PREC_FP64 if mode == "fp64" else abi.PREC_FP32
+    tol = 1e-12 if mode == "fp64" else 1e-5
+    orc = O.Oracle(p, box, traversal=1)
+    ref = col.copy()
+    orc.forces(ref, 0)
+    scale = orc.force_scale(ref)
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.set_tuning(1, 64)
+    md.upload(col)
+    if mode == "list32":
+        md.set_neighbour_list(True, 0.2, 1024)
+    nb, _ = md.bin_info()
+    frc, _ = md.forces()
+    md.close()
+    per_bucket = n / (2 * int(np.prod(nb)))
+    say(f"crowded {mode}: n {n} bins {nb} -> {per_bucket:.0f} particles per bucket")
+    assert per_bucket > 8
+    fref = ref.flat("frc").reshape(-1, 3)
+    err = np.linalg.norm(frc - fref, axis=1)
+    floor = 0.01 * float(np.median(scale)) if mode != "fp64" else 0.0
+    worst = float(np.max(err / (tol * (scale + np.linalg.norm(fref, axis=1) + floor) + 1e-300)))
+    assert worst <= 1.0, worst
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("periodic", [1, 0])
 @pytest.mark.parametrize("nelem", [1, 2])
