@@ -396,6 +396,70 @@ __global__ void __launch_bounds__(WAVE* BINSORT_WAVES) k_binsort(const StepParam
 }
 
 // ======================================================================================
+// Persistent bin sort (neighbour-list mode): keys, bucket scan, scatter and per-bucket sort in
+// one launch, over the same bodies as the separate kernels.  It works only on rebuild passes
+// and otherwise returns at once, so a step that keeps its list pays one launch, not five.
+// The grid is two 256-thread workgroups per CU, all resident (the stream's previous kernel
+// has completed and a CU holds far more), and the phases are separated by grid barriers.  A
+// barrier's wait is bounded: after SYNC_SPIN_LIMIT polls it records ERR_SYNC and the kernel
+// runs to its end without waiting; the host then reports the error and resets the counters.
+// ======================================================================================
+#ifndef SORT_ALL_CU_PER_WG
+#define SORT_ALL_CU_PER_WG 1  // CUs per workgroup of k_sort_all (each workgroup passes a grid barrier with one fence pair)
+#endif
+#define SYNC_SPIN_LIMIT (1u << 18)  // about 0.3 s of polling; a phase takes well under 1 ms
+__device__ inline bool grid_barrier(unsigned* arrive, unsigned target, unsigned* err) {
+  __shared__ int ok_s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    __threadfence();  // this workgroup's writes of the phase, visible device-wide
+    atomicAdd(arrive, 1u);
+    for (unsigned n = 0; __hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++n) {
+      if (n >= SYNC_SPIN_LIMIT) {
+        ok = 0;
+        atomicOr(err, (unsigned)ERR_SYNC);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(8);
+    }
+    __threadfence();  // and the other workgroups' writes visible here
+    ok_s = ok;
+  }
+  __syncthreads();
+  return ok_s != 0;
+}
+
+// sync: [0] arrivals, [1] departures (zero between launches).
+template <typename T>
+__global__ void __launch_bounds__(256) k_sort_all(const StepParams* __restrict__ Pp, Dev S, unsigned* sync) {
+  if (sort_skipped(S)) return;
+  const StepParams& P = *Pp;
+  const unsigned G = gridDim.x;
+  const int np = (S.P + 255) / 256;
+  for (int vb = blockIdx.x; vb < np; vb += (int)G) k_keys_body(Pp, S, vb);
+  bool ok = grid_barrier(sync, G, S.err);
+  const int nt = P.nbuckets > 0 ? (P.nbuckets + SCAN_TILE - 1) / SCAN_TILE : 1;
+  if (ok)
+    for (int vb = blockIdx.x; vb < nt; vb += (int)G) k_scan_local_body(S.count, S.start, S.blocksum, P.nbuckets, vb);
+  ok = ok && grid_barrier(sync, 2 * G, S.err);
+  const int na = P.nbuckets > 0 ? (P.nbuckets + 255) / 256 : 1;
+  if (ok)
+    for (int vb = blockIdx.x; vb < na; vb += (int)G)
+      k_scan_add_body(S.start, S.blocksum, P.nbuckets, S.start + P.nbuckets, vb);
+  ok = ok && grid_barrier(sync, 3 * G, S.err);
+  if (ok)
+    for (int vb = blockIdx.x; vb < np; vb += (int)G) k_scatter_body(S, vb);
+  ok = ok && grid_barrier(sync, 4 * G, S.err);
+  if (ok) k_binsort_body<T>(Pp, S, (int)blockIdx.x, (int)G);
+  __syncthreads();
+  if (ok && threadIdx.x == 0 && atomicAdd(sync + 1, 1u) == G - 1) {  // the last workgroup out
+    sync[0] = 0u;
+    sync[1] = 0u;
+  }
+}
+
+// ======================================================================================
 // Pair forces.  Thread per sorted particle: a wavefront holds 64 consecutive particles of
 // one element type, i.e. a short run of bins along z.  For each source type the stencil
 // (reach per type pair) is walked as (x, y) columns; the z-run of a column is one contiguous

@@ -22,7 +22,7 @@ enum CellStateV : int { CS_DIVIDING = 0, CS_CRAWLING = 1, CS_UNDEFINED = 2 };  /
 // Error bits raised by kernels and collected by the host at synchronisation points.
 enum ErrBit : unsigned { ERR_BIN_OVERFLOW = 1u, ERR_CAPACITY = 2u, ERR_BOND = 4u, ERR_NONFINITE = 8u,
                          ERR_CUTOFF = 16u, ERR_NOISE = 32u, ERR_GUARD = 64u,
-                         ERR_KNN = 128u, ERR_LIST = 256u };
+                         ERR_KNN = 128u, ERR_LIST = 256u, ERR_SYNC = 512u };
 
 // ---------------------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11).  Counter = (id, ts, stream, k), key = seed.
