@@ -1687,12 +1687,15 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
         const int k = S.inv[p];
         int bc[3];
         unpack_bin(P, S.sbin[k], bc);
-        float* sp = (float*)S.spos + 4 * (size_t)k;  // x, y, z of the float4 slot; sigma stays
+        // the whole float4 slot in one aligned store (sigma rewritten with the value the sort
+        // stored), not a partial 12-byte write
+        float l4[4] = {0.f, 0.f, 0.f, (float)sig[e]};
         for (int d = 0; d < D; ++d) {
           double l = xn[d] - bc[d] * P.w[d];
           if (P.box.pbc[d]) l -= P.box.L[d] * rint(l * P.box.invL[d]);
-          sp[d] = (float)l;
+          l4[d] = (float)l;
         }
+        ((float4*)S.spos)[k] = make_float4(l4[0], l4[1], l4[2], l4[3]);
       }
     }
   }
