@@ -1171,7 +1171,8 @@ __device__ inline bool wave_type_slot(const StepParams& P, const Dev& S, int* tp
 // List build of sorted particle k (element type tp), run by the list pair kernel when this
 // pass rebuilds: every pair within its own cut-off plus the skin, found with the culled
 // stencil walk of k_pair32; records the build positions.  Only k's own list is written, so
-// the pair loop of the same lane can read it right after.
+// the pair loop of the same lane can read it right after.  Entries are stored in groups of
+// four (one 16-byte store per group), each source type starting on a group boundary.
 template <int D>
 __device__ inline void build_list_of(const StepParams& P, const Dev& S, int tp, int k) {
   const float4* __restrict__ spos = (const float4*)S.spos;
@@ -1187,10 +1188,13 @@ __device__ inline void build_list_of(const StepParams& P, const Dev& S, int tp, 
   const bool miz = D == 3 && P.reach[2] == 0 && P.box.pbc[2];
   const float skin = P.skin;
   const size_t NP = (size_t)S.P;
-  int cnt = 0;
+  int4* __restrict__ nl4 = (int4*)S.nl;
+  const int ngroups = P.list_max >> 2;
+  int slot = 0;  // next list slot; each source type starts on a group boundary
+  int b0 = 0, b1 = 0, b2 = 0, b3 = 0;  // the group being filled
   bool over = false;
   for (int tq = 0; tq < P.ntypes; ++tq) {
-    const int c0 = cnt;
+    int m = 0;
     const int Rx = P.reach_tt[tp][tq][0];
     if (Rx >= 0) {
       const int Ry = P.reach_tt[tp][tq][1], Rz = D == 3 ? P.reach_tt[tp][tq][2] : 0;
@@ -1248,19 +1252,36 @@ __device__ inline void build_list_of(const StepParams& P, const Dev& S, int tp, 
               if (self_bucket && j == k) continue;
               if (same_species_cells && ((S.smeta[j] >> 3) >> cshift) == cell_p) continue;
               const int oz = D == 3 ? zo[sg] + (u - u0) : 0;
-              if (cnt < P.list_max)
-                S.nl[(size_t)cnt * NP + k] =
-                    (int)(((unsigned)j << 10) | (unsigned)list_code(mix ? 0 : ox, miy ? 0 : oy, miz ? 0 : oz));
-              else over = true;
-              ++cnt;
+              const int e = (int)(((unsigned)j << 10) | (unsigned)list_code(mix ? 0 : ox, miy ? 0 : oy, miz ? 0 : oz));
+              const int s = slot & 3;
+              b0 = s == 0 ? e : b0;
+              b1 = s == 1 ? e : b1;
+              b2 = s == 2 ? e : b2;
+              b3 = s == 3 ? e : b3;
+              ++slot;
+              ++m;
+              if (s == 3) {  // group complete: one 16-byte store
+                const int g = (slot >> 2) - 1;
+                if (g < ngroups) nl4[(size_t)g * NP + k] = make_int4(b0, b1, b2, b3);
+                else over = true;
+              }
             }
           }
         }
       }
     }
-    S.nlt[(size_t)tq * NP + k] = (cnt < P.list_max ? cnt : P.list_max) - (c0 < P.list_max ? c0 : P.list_max);
+    if (slot & 3) {  // partial last group of this source type (its unused slots are never read)
+      const int g = slot >> 2;
+      if (g < ngroups) nl4[(size_t)g * NP + k] = make_int4(b0, b1, b2, b3);
+      else over = true;
+      slot = (slot + 3) & ~3;
+    }
+    S.nlt[(size_t)tq * NP + k] = m;
   }
-  if (over) atomicOr(S.err, (unsigned)ERR_LIST);
+  if (over) {  // capacity error: an empty list keeps the pair loop inside the allocation
+    for (int tq = 0; tq < P.ntypes; ++tq) S.nlt[(size_t)tq * NP + k] = 0;
+    atomicOr(S.err, (unsigned)ERR_LIST);
+  }
 }
 
 // Bin-offset table of the list kernels, in LDS: entry code -> ((float)ox * wx, (float)oy * wy,
@@ -1268,9 +1289,6 @@ __device__ inline void build_list_of(const StepParams& P, const Dev& S, int tp, 
 // block before any wave leaves.
 constexpr int LIST_CODES = 729;
 constexpr int LIST_BLOCK = 256;
-#ifndef LIST_UNROLL
-#define LIST_UNROLL 4
-#endif
 #ifndef LIST_WAVES_PER_EU
 #define LIST_WAVES_PER_EU 6
 #endif
@@ -1322,14 +1340,16 @@ __device__ __forceinline__ float sc_term32(const Law32& L, float ssum, float r2)
   return f0;
 }
 
-// A list word (plain cached load: non-temporal loads of the list measured slower).
-__device__ __forceinline__ unsigned list_word(const char* __restrict__ nlb, unsigned off) {
-  return *(const unsigned*)(nlb + off);
+// A group of four list entries (plain cached load: non-temporal loads of the list measured
+// slower).
+__device__ __forceinline__ int4 list_group(const char* __restrict__ nlb, unsigned off) {
+  return *(const int4*)(nlb + off);
 }
 
-// One source-type segment of a particle's list: n entries from byte offset `off` of the list
-// (rows `stride` bytes apart), in list order (the summation order is the list order).  LC is
-// the segment's law class, MI whether any dimension uses the single-bin minimum image.
+// One source-type segment of a particle's list: n entries in groups of four from byte offset
+// `off` of the list (groups `stride` bytes apart), in list order (the summation order is the
+// list order).  LC is the segment's law class, MI whether any dimension uses the single-bin
+// minimum image.
 template <int D, int LC, bool MI, bool SH>
 __device__ __forceinline__ void list_segment(const StepParams& P, const Dev& S, const Law32& Lw, const float4* tab,
                                              const float4& me, int k, int pidx, uint32_t ts, float cadh, int n,
@@ -1376,33 +1396,34 @@ __device__ __forceinline__ void list_segment(const StepParams& P, const Dev& S, 
     if (D == 3) Fz = fmaf(fs, dz, Fz);
   };
   int i = 0;
-  // LIST_UNROLL entries per round: their list words and positions are loaded before any is
-  // used, so that many independent gathers are in flight per lane
-  constexpr int U = LIST_UNROLL;
-  for (; i + U <= n; i += U) {
-    unsigned e[U];
-    float4 r[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) e[u] = list_word(nlb, off + (unsigned)u * stride);
-    off += (unsigned)U * stride;
-#pragma unroll
-    for (int u = 0; u < U; ++u) r[u] = spos[e[u] >> 10];
-    float4 o[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) o[u] = tab[e[u] & 1023u];
-#pragma unroll
-    for (int u = 0; u < U; ++u) term(e[u], r[u], o[u]);
-  }
-  for (; i < n; ++i) {
-    const unsigned e = list_word(nlb, off);
+  // one group per round: its four positions are loaded before any is used, so that four
+  // independent gathers are in flight per lane
+  for (; i + 4 <= n; i += 4) {
+    const int4 g = list_group(nlb, off);
     off += stride;
-    term(e, spos[e >> 10], tab[e & 1023u]);
+    const unsigned e[4] = {(unsigned)g.x, (unsigned)g.y, (unsigned)g.z, (unsigned)g.w};
+    float4 r[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) r[u] = spos[e[u] >> 10];
+    float4 o[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) o[u] = tab[e[u] & 1023u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) term(e[u], r[u], o[u]);
+  }
+  if (i < n) {  // last, partial group: 1 to 3 entries
+    const int4 g = list_group(nlb, off);
+    off += stride;
+    const unsigned e[3] = {(unsigned)g.x, (unsigned)g.y, (unsigned)g.z};
+    term(e[0], spos[e[0] >> 10], tab[e[0] & 1023u]);
+    if (i + 1 < n) term(e[1], spos[e[1] >> 10], tab[e[1] & 1023u]);
+    if (i + 2 < n) term(e[2], spos[e[2] >> 10], tab[e[2] & 1023u]);
   }
 }
 
 // Pair forces over the neighbour list (every step), after rebuilding the particle's list when
 // this pass's rebuild flag is set.  Same law code, exclusions, guard band and per-type wave
-// layout as k_pair32.  List rows are addressed with 32-bit byte offsets
+// layout as k_pair32.  List groups are addressed with 32-bit byte offsets
 // (list_max * P * 4 < 2^32, checked by the host).
 template <int D>
 __device__ inline void k_pair_list_body(const StepParams* __restrict__ Pp, const Dev& S) {
@@ -1422,8 +1443,8 @@ __device__ inline void k_pair_list_body(const StepParams* __restrict__ Pp, const
   const bool mi = (P.reach[0] == 0 && P.box.pbc[0]) || (P.reach[1] == 0 && P.box.pbc[1]) ||
                   (D == 3 && P.reach[2] == 0 && P.box.pbc[2]);
   const size_t NP = (size_t)S.P;
-  const unsigned stride = 4u * (unsigned)S.P;
-  unsigned off = 4u * (unsigned)k;
+  const unsigned stride = 16u * (unsigned)S.P;
+  unsigned off = 16u * (unsigned)k;
   float Fx = 0.f, Fy = 0.f, Fz = 0.f, pr = 0.f;
   int nguard = 0;
   for (int tq = 0; tq < P.ntypes; ++tq) {

@@ -68,7 +68,8 @@ struct Dev {
                       // re-sorts and rebuilds the list (set by the skin test, division and
                       // host-initiated sorts; the update of step u clears word u & 1, whose
                       // pass has run)
-  int* nl;            // [list_max][P] (sorted index << 9 | bin offset code), per type contiguous
+  int* nl;            // [list_max / 4][P][4] groups of (sorted index << 10 | bin offset code);
+                      // each source type starts on a group boundary
   int* nlt;           // [ntypes][P] entries per source type
   float* xref;        // [3][P] positions at the last build (particle order)
 };

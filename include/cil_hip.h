@@ -258,6 +258,9 @@ int cil_hip_set_timing(cil_ctx* ctx, int on);
 
 /* Neighbour-list mode (project addition; FP32 force precision, fewer than 2^22 particle slots,
  * max_neighbours * particle slots * 4 bytes of list below 4 GiB, 8 <= max_neighbours <= 4096).
+ * max_neighbours is the list slots per particle, rounded up to a multiple of 4; the entries of
+ * each source element type start on a 4-slot boundary, so a particle holds at least
+ * max_neighbours - 3 * (element types - 1) entries before the step fails with CIL_E_CAPACITY.
  * Pairs within their cut-off + skin are listed at a build; each step evaluates only the list
  * and re-sorts / rebuilds when a particle has moved more than skin / 2 since the build, when
  * cells divide or are removed, or after an upload.  The pair set equals the stencil kernel's
