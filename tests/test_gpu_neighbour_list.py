@@ -186,6 +186,27 @@ def test_list_overflow_is_reported(gpu):
     md.close()
 
 
+def test_list_capacity_in_groups_of_four(gpu):
+    # max_neighbours counts list slots in whole groups of four: 157 is taken as 160.  A capacity
+    # that holds every list gives the same lists, hence bit-identical steps, as a far larger one
+    # (two source types here, each starting on a group boundary); rebuilds every few steps.
+    p, box, col = _relaxed(4096, 6)
+    out = []
+    for m in (157, 160, 1024):
+        md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+        md.upload(col)
+        md.set_neighbour_list(True, 0.05, m)
+        md.forces(want=False)
+        assert md.step(40, 0) == 40
+        c = col.copy()
+        md.download(c)
+        md.close()
+        out.append(c.flat("pos"))
+        say("capacity", m, "done")
+    assert np.array_equal(out[0], out[1])
+    assert np.array_equal(out[1], out[2])
+
+
 def test_list_mode_rejects_fp64(gpu):
     p, box, col = configs.plain_2d_sc(n_cells=50, L=12.0, seed=3)
     md = HipMD(p, box, [sp["capacity"] for sp in col.species])
