@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "kernels.hpp"
@@ -1168,13 +1169,32 @@ __device__ inline bool wave_type_slot(const StepParams& P, const Dev& S, int* tp
   return k0 + lane < k1;
 }
 
+// Wave-wide minimum / maximum over all 64 lanes (every lane must be active).
+__device__ __forceinline__ int wave_min_all(int v) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, WAVE));
+  return v;
+}
+__device__ __forceinline__ int wave_max_all(int v) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, WAVE));
+  return v;
+}
+// Candidate window of the list build: sorted positions staged per wave in LDS.
+constexpr int BUILD_WIN = 160;
+
 // List build of sorted particle k (element type tp), run by the list pair kernel when this
 // pass rebuilds: every pair within its own cut-off plus the skin, found with the culled
 // stencil walk of k_pair32; records the build positions.  Only k's own list is written, so
 // the pair loop of the same lane can read it right after.  Entries are stored in groups of
 // four (one 16-byte store per group), each source type starting on a group boundary.
+// The lanes of a wave (consecutive particles, mostly one z-run of one column) walk
+// overlapping candidate ranges in each stencil column: when all 64 lanes hold a particle and
+// the union of their ranges fits BUILD_WIN, the wave loads it once, coalesced, into its LDS
+// window `win` and the lanes read their candidates there instead of gathering each from
+// memory.  The candidates, their order and the arithmetic are the same either way.
 template <int D>
-__device__ inline void build_list_of(const StepParams& P, const Dev& S, int tp, int k) {
+__device__ inline void build_list_of(const StepParams& P, const Dev& S, int tp, int k, float4* win) {
   const float4* __restrict__ spos = (const float4*)S.spos;
   const int* __restrict__ start = S.start;
   const int nE = P.nE, cshift = nE - 1, spec_p = tp / nE;
@@ -1188,6 +1208,8 @@ __device__ inline void build_list_of(const StepParams& P, const Dev& S, int tp, 
   const bool miz = D == 3 && P.reach[2] == 0 && P.box.pbc[2];
   const float skin = P.skin;
   const size_t NP = (size_t)S.P;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const bool full = __ballot(1) == ~0ull;  // all lanes present: the window can be shared
   int4* __restrict__ nl4 = (int4*)S.nl;
   const int ngroups = P.list_max >> 2;
   int slot = 0;  // next list slot; each source type starts on a group boundary
@@ -1204,69 +1226,98 @@ __device__ inline void build_list_of(const StepParams& P, const Dev& S, int tp, 
       const float rcp = (fmaf(rca, me.w + (float)P.sigma_bound[tq], rcb) + skin) * 1.001f + 1e-6f;
       const float rcp2 = rcp * rcp;
       int z0[2], z1[2], zo[2];
+      // the (ox, oy) loops are wave-uniform; a lane whose column is culled or outside the
+      // box walks no segment but still takes part in the window's reduction and load
       for (int ox = -Rx; ox <= Rx; ++ox) {
         int nx = bc[0] + ox;
+        bool okx = true;
         if (P.box.pbc[0]) nx = nx < 0 ? nx + P.ns[0] : (nx >= P.ns[0] ? nx - P.ns[0] : nx);
-        else if (nx < 0 || nx >= P.ns[0]) continue;
+        else okx = nx >= 0 && nx < P.ns[0];
         const float qx = me.x - (float)ox * wx;
         const float gx = mix ? 0.0f : (ox > 0 ? (float)ox * wx - me.x : (ox < 0 ? me.x - (float)(ox + 1) * wx : 0.0f));
         const float gx2 = gx > 0.0f ? gx * gx : 0.0f;
-        if (gx2 > rcp2) continue;
+        okx = okx && !(gx2 > rcp2);
         for (int oy = -Ry; oy <= Ry; ++oy) {
           int ny = bc[1] + oy;
+          bool ok = okx;
           if (P.box.pbc[1]) ny = ny < 0 ? ny + P.ns[1] : (ny >= P.ns[1] ? ny - P.ns[1] : ny);
-          else if (ny < 0 || ny >= P.ns[1]) continue;
+          else ok = ok && ny >= 0 && ny < P.ns[1];
           const float qy = me.y - (float)oy * wy;
           const float gy = miy ? 0.0f : (oy > 0 ? (float)oy * wy - me.y : (oy < 0 ? me.y - (float)(oy + 1) * wy : 0.0f));
           const float rem = rcp2 - gx2 - (gy > 0.0f ? gy * gy : 0.0f);
-          if (rem < 0.0f) continue;
+          ok = ok && !(rem < 0.0f);
           int zlo = -Rz, zhi = Rz;
-          if (D == 3 && !miz) {
+          if (D == 3 && !miz && ok) {
             const float sr = sqrtf(rem);
             const int a = (int)floorf((me.z + sr) / wz), b = (int)ceilf((me.z - sr) / wz) - 1;
             zhi = a < Rz ? a : Rz;
             zlo = b > -Rz ? b : -Rz;
-            if (zlo > zhi) continue;
+            ok = zlo <= zhi;
           }
-          const int nseg = z_segments_range(P, bc[2], zlo, zhi, z0, z1, zo);
+          const int nseg = ok ? z_segments_range(P, bc[2], zlo, zhi, z0, z1, zo) : 0;
+          int u0s[2] = {0, 0}, j0s[2] = {0, 0}, j1s[2] = {0, 0};
+          int lo = 0x7fffffff, hi = (int)0x80000000;
           for (int sg = 0; sg < nseg; ++sg) {
-            const int u0 = bucket_of(P, tq, nx, ny, z0[sg]);
-            const int u1 = bucket_of(P, tq, nx, ny, z1[sg]);
-            const int j1 = start[u1 + 1];
-            int u = u0, jend = start[u0 + 1];
-            float qz = me.z - (float)zo[sg] * wz;
-            for (int j = start[u0]; j < j1; ++j) {
-              while (j >= jend) {
-                ++u;
-                jend = start[u + 1];
-                qz -= wz;
-              }
-              const float4 r = spos[j];
-              float dx = r.x - qx, dy = r.y - qy, dz = D == 3 ? r.z - qz : 0.0f;
-              if (mix) dx -= (float)P.box.L[0] * rintf(dx * (float)P.box.invL[0]);
-              if (miy) dy -= (float)P.box.L[1] * rintf(dy * (float)P.box.invL[1]);
-              if (miz) dz -= (float)P.box.L[2] * rintf(dz * (float)P.box.invL[2]);
-              const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-              const float rl = (fmaf(rca, r.w + me.w, rcb) + skin) * 1.001f;
-              if (r2 > rl * rl) continue;
-              if (self_bucket && j == k) continue;
-              if (same_species_cells && ((S.smeta[j] >> 3) >> cshift) == cell_p) continue;
-              const int oz = D == 3 ? zo[sg] + (u - u0) : 0;
-              const int e = (int)(((unsigned)j << 10) | (unsigned)list_code(mix ? 0 : ox, miy ? 0 : oy, miz ? 0 : oz));
-              const int s = slot & 3;
-              b0 = s == 0 ? e : b0;
-              b1 = s == 1 ? e : b1;
-              b2 = s == 2 ? e : b2;
-              b3 = s == 3 ? e : b3;
-              ++slot;
-              ++m;
-              if (s == 3) {  // group complete: one 16-byte store
-                const int g = (slot >> 2) - 1;
-                if (g < ngroups) nl4[(size_t)g * NP + k] = make_int4(b0, b1, b2, b3);
-                else over = true;
-              }
+            u0s[sg] = bucket_of(P, tq, nx, ny, z0[sg]);
+            j0s[sg] = start[u0s[sg]];
+            j1s[sg] = start[bucket_of(P, tq, nx, ny, z1[sg]) + 1];
+            lo = j0s[sg] < lo ? j0s[sg] : lo;
+            hi = j1s[sg] > hi ? j1s[sg] : hi;
+          }
+          bool staged = false;
+          int wlo = 0;
+          if (full) {
+            wlo = wave_min_all(lo);
+            const int whi = wave_max_all(hi);
+            staged = whi > wlo && whi - wlo <= BUILD_WIN;
+            if (staged) {
+              for (int c = lane; c < whi - wlo; c += WAVE) win[c] = spos[wlo + c];
+              wave_lds_sync();
             }
           }
+          // one compiled walk per candidate source, so that each issues a single kind of load
+          auto walk = [&](auto from_win) {
+            constexpr bool FROM_WIN = decltype(from_win)::value;
+            for (int sg = 0; sg < nseg; ++sg) {
+              const int j1 = j1s[sg];
+              int u = u0s[sg], jend = start[u + 1];
+              float qz = me.z - (float)zo[sg] * wz;
+              for (int j = j0s[sg]; j < j1; ++j) {
+                while (j >= jend) {
+                  ++u;
+                  jend = start[u + 1];
+                  qz -= wz;
+                }
+                const float4 r = FROM_WIN ? win[j - wlo] : spos[j];
+                float dx = r.x - qx, dy = r.y - qy, dz = D == 3 ? r.z - qz : 0.0f;
+                if (mix) dx -= (float)P.box.L[0] * rintf(dx * (float)P.box.invL[0]);
+                if (miy) dy -= (float)P.box.L[1] * rintf(dy * (float)P.box.invL[1]);
+                if (miz) dz -= (float)P.box.L[2] * rintf(dz * (float)P.box.invL[2]);
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                const float rl = (fmaf(rca, r.w + me.w, rcb) + skin) * 1.001f;
+                if (r2 > rl * rl) continue;
+                if (self_bucket && j == k) continue;
+                if (same_species_cells && ((S.smeta[j] >> 3) >> cshift) == cell_p) continue;
+                const int oz = D == 3 ? zo[sg] + (u - u0s[sg]) : 0;
+                const int e = (int)(((unsigned)j << 10) | (unsigned)list_code(mix ? 0 : ox, miy ? 0 : oy, miz ? 0 : oz));
+                const int s = slot & 3;
+                b0 = s == 0 ? e : b0;
+                b1 = s == 1 ? e : b1;
+                b2 = s == 2 ? e : b2;
+                b3 = s == 3 ? e : b3;
+                ++slot;
+                ++m;
+                if (s == 3) {  // group complete: one 16-byte store
+                  const int g = (slot >> 2) - 1;
+                  if (g < ngroups) nl4[(size_t)g * NP + k] = make_int4(b0, b1, b2, b3);
+                  else over = true;
+                }
+              }
+            }
+          };
+          if (staged) walk(std::true_type{});
+          else walk(std::false_type{});
+          if (staged) wave_lds_sync();  // every lane done with the window before it is rewritten
         }
       }
     }
@@ -1432,11 +1483,12 @@ template <int D>
 __device__ inline void k_pair_list_body(const StepParams* __restrict__ Pp, const Dev& S) {
   const StepParams& P = *Pp;
   __shared__ float4 tab[LIST_CODES];
+  __shared__ float4 bwin[LIST_BLOCK / WAVE][BUILD_WIN];  // per-wave candidate windows of the build
   list_offsets_fill(P, tab);
   const bool rebuild = *rebuild_flag(S, step_ts(S)) != 0u;
   int tp, k;
   if (!wave_type_slot(P, S, &tp, &k)) return;
-  if (rebuild) build_list_of<D>(P, S, tp, k);
+  if (rebuild) build_list_of<D>(P, S, tp, k, bwin[threadIdx.x / WAVE]);
   const float4* __restrict__ spos = (const float4*)S.spos;
   const uint32_t ts = (uint32_t)step_ts(S);
   const int nE = P.nE, spec_p = tp / nE;
