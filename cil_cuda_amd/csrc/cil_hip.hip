@@ -32,12 +32,8 @@ using namespace cil;
 #define WAVE 64
 #define GUARD_SLOTS 4
 
-// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8) and each XCD
-// has its own L2.  Kernels whose consecutive workgroups work on neighbouring bins renumber
-// them so that each XCD processes one contiguous range: a bin's neighbour data is then
-// fetched into one L2 instead of eight.  A bijection of [0, nblk).
-// Neighbour-list mode gates the sort kernels on the device rebuild flag (S.rebuild is null
-// outside list mode): steps that keep the list skip the whole re-sort with no host round trip.
+// Neighbour-list mode gates the sort on the device rebuild flag (S.rebuild is null outside
+// list mode): steps that keep the list skip the whole re-sort with no host round trip.
 __device__ inline bool sort_skipped(const Dev& S) { return S.rebuild != nullptr && *rebuild_flag(S, step_ts(S)) == 0u; }
 
 // Gate of the bucket scan: closed when list mode skips this pass's sort (same test as
@@ -50,6 +46,10 @@ struct SortGate {
 __device__ inline bool gate_closed(const SortGate& g) { return g.flags && g.flags[(*g.ts + g.off) & 1] == 0u; }
 
 #define NUM_XCD 8
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8) and each XCD
+// has its own L2.  Kernels whose consecutive workgroups work on neighbouring bins renumber
+// them so that each XCD processes one contiguous range: a bin's neighbour data is then
+// fetched into one L2 instead of eight.  A bijection of [0, nblk).
 __device__ inline int xcd_block(int b, int nblk) {
   const int q = nblk / NUM_XCD, r = nblk % NUM_XCD, x = b % NUM_XCD;
   return x * q + (x < r ? x : r) + b / NUM_XCD;
@@ -399,7 +399,7 @@ __global__ void __launch_bounds__(WAVE* BINSORT_WAVES) k_binsort(const StepParam
 // ======================================================================================
 // Persistent bin sort (neighbour-list mode): keys, bucket scan, scatter and per-bucket sort in
 // one launch, over the same bodies as the separate kernels.  It works only on rebuild passes
-// and otherwise returns at once, so a step that keeps its list pays one launch, not five.
+// and otherwise returns at once, so a step that keeps its list pays one launch.
 // The grid is one 256-thread workgroup per CU, all resident (a CU holds several once the
 // stream's previous kernel has drained), and the phases are separated by grid barriers.  A
 // barrier's wait is bounded: after SYNC_SPIN_LIMIT polls it records ERR_SYNC and the kernel
