@@ -113,6 +113,57 @@ def test_pairs_bit_exact(gpu, dim, periodic, nelem, law):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("mode", ["fp64", "fp32", "list32"])
+def test_empty_and_ragged_species(gpu, mode):
+    """Ragged input: species 0 empty (its region allocated), species 1 populated.  Forces meet
+    the mode's bar against the oracle, 40 steps match the oracle's trajectory (FP64: 1e-9 sigma;
+    FP32 modes: 1e-5 sigma), and a colony with every species empty steps without error and
+    stays empty."""
+    p, box, col = _random_colony(3, 1, 600, 8.0, 1, abi.LAW_SC, seed=23)
+    p.precision = abi.PREC_FP64 if mode == "fp64" else abi.PREC_FP32
+    col.species[0]["ncells"] = 0
+    assert col.ncells() == [0, 300]
+    tol = 1e-12 if mode == "fp64" else 1e-5
+    orc = O.Oracle(p, box, traversal=1)
+    ref = col.copy()
+    orc.forces(ref, 0)
+    scale = orc.force_scale(ref)
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    if mode == "list32":
+        md.set_neighbour_list(True, 0.2, 256)
+    frc, _ = md.forces()
+    fref = ref.flat("frc").reshape(-1, 3)
+    assert frc.shape == fref.shape == (300, 3)
+    err = np.linalg.norm(frc - fref, axis=1)
+    floor = 0.01 * float(np.median(scale)) if mode != "fp64" else 0.0
+    worst = float(np.max(err / (tol * (scale + np.linalg.norm(fref, axis=1) + floor) + 1e-300)))
+    say(f"ragged {mode}: worst force err/bound {worst:.3f}")
+    assert worst <= 1.0, worst
+    assert orc.step(ref, 40, 0) == 40
+    assert md.step(40, 0) == 40
+    got = col.copy()
+    md.download(got)
+    md.close()
+    assert got.ncells() == ref.ncells() == [0, 300]
+    d = _max_disp_err(ref, got, box)
+    say(f"ragged {mode}: 40-step max displacement difference {d:.3e}")
+    assert d <= (1e-9 if mode == "fp64" else 1e-5)
+
+    for sp in col.species:
+        sp["ncells"] = 0
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    if mode == "list32":
+        md.set_neighbour_list(True, 0.2, 256)
+    md.forces(want=False)
+    assert md.step(10, 0) == 10
+    got = col.copy()
+    md.download(got)
+    md.close()
+    assert got.ncells() == [0, 0]
+
+
 def _force_parity(p, box, col, tol, ts=0):
     """Worst err/bound over particles.  In FP32 mode the bound has an absolute floor of 1% of
     the median force scale: a grazing contact (S_p -> 0) carries FP32's absolute rounding
@@ -261,6 +312,28 @@ def test_linear_growth_divisions_and_drain(gpu):
     assert np.array_equal(ref.species[0]["t0"][:n], got.species[0]["t0"][:n])
     assert np.allclose(ref.species[0]["rest_len"][:n], got.species[0]["rest_len"][:n], atol=1e-12)
     assert _max_disp_err(ref, got, box) <= 1e-9
+
+
+def test_division_beyond_capacity_is_reported(gpu):
+    """A full species region: a division that would overflow it is not applied, the step
+    returns a capacity error with a message, and the cell count stays at the capacity with
+    finite positions."""
+    say("division beyond capacity")
+    p, box, col = configs.channel_2d_hz(width=12.0, length=30.0, capacity=4, dt=1e-3, tau=0.05)
+    md = HipMD(p, box, [4])
+    md.upload(col)
+    md.forces(want=False)
+    ts = 0
+    with pytest.raises(abi.CilError) as ei:
+        for _ in range(40):
+            ts = md.step(25, ts)
+    say("capacity error:", ei.value)
+    assert "capacity" in str(ei.value)
+    got = col.copy()
+    md.download(got)
+    md.close()
+    assert got.ncells() == [4]
+    assert np.all(np.isfinite(got.flat("pos")))
 
 
 def test_drain_removal_order(gpu):
