@@ -409,14 +409,29 @@ __global__ void __launch_bounds__(WAVE* BINSORT_WAVES) k_binsort(const StepParam
 #define SORT_ALL_CU_PER_WG 1  // CUs per workgroup of k_sort_all (each workgroup passes a grid barrier with one fence pair)
 #endif
 #define SYNC_SPIN_LIMIT (1u << 18)  // about 0.3 s of polling; a phase takes well under 1 ms
-__device__ inline bool grid_barrier(unsigned* arrive, unsigned target, unsigned* err) {
+// Barrier counters, each on its own 128-byte line: arrivals of the workgroups b with
+// b % NUM_XCD == x at line x, arrivals of whole groups at line NUM_XCD; departures likewise
+// from line NUM_XCD + 1.  Arrivals on one word serialise, so each workgroup adds to its group's
+// word and only the last of a group adds to the shared one: NUM_XCD arrivals per phase there
+// instead of one per workgroup.  All words are zero between launches.
+constexpr int SYNC_LINE = 32;
+constexpr int SYNC_WORDS = 2 * (NUM_XCD + 1) * SYNC_LINE;
+__device__ inline unsigned sync_group_size(unsigned G, unsigned x) { return x < G ? (G - x + NUM_XCD - 1) / NUM_XCD : 0u; }
+// Phase `phase` (1, 2, ...) of a grid barrier over gridDim.x resident workgroups.
+__device__ inline bool grid_barrier(unsigned* sync, unsigned phase, unsigned* err) {
   __shared__ int ok_s;
   __syncthreads();
   if (threadIdx.x == 0) {
+    const unsigned G = gridDim.x, x = blockIdx.x % NUM_XCD;
+    const unsigned groups = G < (unsigned)NUM_XCD ? G : (unsigned)NUM_XCD;
+    unsigned* all = sync + NUM_XCD * SYNC_LINE;
     int ok = 1;
     __threadfence();  // this workgroup's writes of the phase, visible device-wide
-    atomicAdd(arrive, 1u);
-    for (unsigned n = 0; __hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++n) {
+    if (atomicAdd(sync + x * SYNC_LINE, 1u) == phase * sync_group_size(G, x) - 1u) {
+      __threadfence();  // the group's writes, passed on with its arrival
+      atomicAdd(all, 1u);
+    }
+    for (unsigned n = 0; __hip_atomic_load(all, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < phase * groups; ++n) {
       if (n >= SYNC_SPIN_LIMIT) {
         ok = 0;
         atomicOr(err, (unsigned)ERR_SYNC);
@@ -430,8 +445,18 @@ __device__ inline bool grid_barrier(unsigned* arrive, unsigned target, unsigned*
   __syncthreads();
   return ok_s != 0;
 }
+// After the last phase: the last workgroup to leave zeroes every counter for the next launch.
+__device__ inline void grid_depart(unsigned* sync) {
+  if (threadIdx.x != 0) return;
+  const unsigned G = gridDim.x, x = blockIdx.x % NUM_XCD;
+  const unsigned groups = G < (unsigned)NUM_XCD ? G : (unsigned)NUM_XCD;
+  unsigned* dep = sync + (NUM_XCD + 1) * SYNC_LINE;
+  if (atomicAdd(dep + x * SYNC_LINE, 1u) == sync_group_size(G, x) - 1u &&
+      atomicAdd(dep + NUM_XCD * SYNC_LINE, 1u) == groups - 1u)
+    for (int w = 0; w < 2 * (NUM_XCD + 1); ++w) sync[w * SYNC_LINE] = 0u;
+}
 
-// sync: [0] arrivals, [1] departures (zero between launches).
+// sync: SYNC_WORDS barrier counters (zero between launches).
 template <typename T>
 __global__ void __launch_bounds__(256) k_sort_all(const StepParams* __restrict__ Pp, Dev S, unsigned* sync) {
   if (sort_skipped(S)) return;
@@ -439,25 +464,22 @@ __global__ void __launch_bounds__(256) k_sort_all(const StepParams* __restrict__
   const unsigned G = gridDim.x;
   const int np = (S.P + 255) / 256;
   for (int vb = blockIdx.x; vb < np; vb += (int)G) k_keys_body(Pp, S, vb);
-  bool ok = grid_barrier(sync, G, S.err);
+  bool ok = grid_barrier(sync, 1, S.err);
   const int nt = P.nbuckets > 0 ? (P.nbuckets + SCAN_TILE - 1) / SCAN_TILE : 1;
   if (ok)
     for (int vb = blockIdx.x; vb < nt; vb += (int)G) k_scan_local_body(S.count, S.start, S.blocksum, P.nbuckets, vb);
-  ok = ok && grid_barrier(sync, 2 * G, S.err);
+  ok = ok && grid_barrier(sync, 2, S.err);
   const int na = P.nbuckets > 0 ? (P.nbuckets + 255) / 256 : 1;
   if (ok)
     for (int vb = blockIdx.x; vb < na; vb += (int)G)
       k_scan_add_body(S.start, S.blocksum, P.nbuckets, S.start + P.nbuckets, vb);
-  ok = ok && grid_barrier(sync, 3 * G, S.err);
+  ok = ok && grid_barrier(sync, 3, S.err);
   if (ok)
     for (int vb = blockIdx.x; vb < np; vb += (int)G) k_scatter_body(S, vb);
-  ok = ok && grid_barrier(sync, 4 * G, S.err);
+  ok = ok && grid_barrier(sync, 4, S.err);
   if (ok) k_binsort_body<T>(Pp, S, (int)blockIdx.x, (int)G);
   __syncthreads();
-  if (ok && threadIdx.x == 0 && atomicAdd(sync + 1, 1u) == G - 1) {  // the last workgroup out
-    sync[0] = 0u;
-    sync[1] = 0u;
-  }
+  if (ok) grid_depart(sync);
 }
 
 // ======================================================================================
