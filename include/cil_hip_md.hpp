@@ -144,6 +144,15 @@ class Session {
     run_assert(cil_hip_download(ctx_, v.data(), (int)v.size(), need.data()), ctx_, "cil_hip_download");
     col_.pull_counts(v);
   }
+  // Bins per cut-off and pair-kernel block (project addition; cil_hip_set_tuning)
+  void setTuning(int bins_per_cutoff, int block) {
+    run_assert(cil_hip_set_tuning(ctx_, bins_per_cutoff, block), ctx_, "cil_hip_set_tuning");
+  }
+  // Neighbour-list mode for FP32 forces (project addition; cil_hip_set_neighbour_list)
+  void setNeighbourList(bool on, double skin = 0.3, int max_neighbours = 160) {
+    run_assert(cil_hip_set_neighbour_list(ctx_, on ? 1 : 0, skin, max_neighbours), ctx_,
+               "cil_hip_set_neighbour_list");
+  }
   cil_ctx* ctx() { return ctx_; }
 
  private:

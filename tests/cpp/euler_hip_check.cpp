@@ -47,7 +47,7 @@ int main() {
   box.length[0] = box.length[1] = L;
   box.periodic[0] = box.periodic[1] = 1;
 
-  cil::hip::CellColony gpu(2, 2, {n}), cpu(2, 2, {n}), b1(2, 2, {n}), b2(2, 2, {n});
+  cil::hip::CellColony gpu(2, 2, {n}), cpu(2, 2, {n}), b1(2, 2, {n}), b2(2, 2, {n}), l32(2, 2, {n});
   std::mt19937_64 rng(5);
   std::uniform_real_distribution<double> U(0.0, 1.0);
   for (int c = 0; c < n; ++c) {
@@ -67,6 +67,7 @@ int main() {
   }
   b1 = gpu;  // batch members: the same colony (seed 7) and a second one with seed 8
   b2 = gpu;
+  l32 = gpu;  // FP32 forces with neighbour lists
   try {
     cil::hip::Session session(p, box, gpu);
     session.upload();
@@ -119,7 +120,29 @@ int main() {
         }
     std::printf("batch: colony 0 %s the single session, colony 1 (other seed) %s\n",
                 same ? "equals" : "DIFFERS FROM", differs ? "differs" : "DOES NOT DIFFER");
-    return (ts == nsteps && err <= 1e-9 && same && differs && tsb[0] == nsteps && tsb[1] == nsteps) ? 0 : 1;
+    // FP32 forces over neighbour lists through the same layer, against the FP64 oracle
+    cil_params p32 = p;
+    p32.precision = CIL_PREC_FP32;
+    cil::hip::Session s3(p32, box, l32);
+    s3.upload();
+    s3.setNeighbourList(true, 0.3, 160);
+    s3.calculForcesPressure();
+    int ts3 = 0;
+    s3.steps(nsteps, ts3);
+    s3.download();
+    double err32 = 0.0;
+    for (int c = 0; c < n; ++c)
+      for (int e = 0; e < 2; ++e)
+        for (int d = 0; d < 2; ++d) {
+          double x = l32.pos(0, c, e, d) - cpu.pos(0, c, e, d);
+          x -= L * std::round(x / L);
+          err32 = std::fmax(err32, std::fabs(x));
+        }
+    std::printf("FP32 neighbour-list session: max position difference %.3e\n", err32);
+    return (ts == nsteps && err <= 1e-9 && same && differs && tsb[0] == nsteps && tsb[1] == nsteps &&
+            ts3 == nsteps && err32 <= 1e-5)
+               ? 0
+               : 1;
   } catch (const std::runtime_error& e) {
     if (std::strstr(e.what(), "status -2")) {
       std::printf("no GPU device: %s\n", e.what());

@@ -1,5 +1,6 @@
 """The C++ host layer (include/cil_hip_md.hpp: CellColony, Session, EulerHIP as an
-MDIntegrator) builds against the C ABI and, on a GPU, matches the oracle over 200 steps."""
+MDIntegrator) builds against the C ABI and, on a GPU, matches the oracle over 200 steps
+(FP64 within 1e-9 sigma; FP32 over neighbour lists within 1e-5 sigma)."""
 import os
 import subprocess
 
