@@ -1054,7 +1054,6 @@ __device__ inline void k_pair32_body(const StepParams* __restrict__ Pp, const De
   const float wx = (float)P.w[0], wy = (float)P.w[1], wz = (float)P.w[2];
   const bool mix = P.reach[0] == 0 && P.box.pbc[0], miy = P.reach[1] == 0 && P.box.pbc[1];
   const bool miz = D == 3 && P.reach[2] == 0 && P.box.pbc[2];
-  const bool shifted = P.force_shifted != 0;
   float Fx = 0.f, Fy = 0.f, Fz = 0.f, pr = 0.f;
   int nguard = 0;
   for (int tq = 0; tq < P.ntypes; ++tq) {
