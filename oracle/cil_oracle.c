@@ -326,6 +326,7 @@ void or_set_rng(or_ws* w, int mode, uint32_t seed) {
   w->rng_mode = mode;
   if (mode == OR_RNG_DSFMT) or_dsfmt_init(&w->g, seed);
 }
+double or_rng_draw(or_ws* w, int kind) { return kind == 1 ? or_dsfmt_normal(&w->g) : or_dsfmt_close_open(&w->g); }
 void or_set_traversal(or_ws* w, int mode) { w->traversal = mode; }
 int or_set_threads(or_ws* w, int n) {
   w->nthreads = n < 1 ? 1 : n;

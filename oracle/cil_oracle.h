@@ -50,6 +50,10 @@ typedef struct or_ws or_ws;
 or_ws* or_create(const cil_params* p, const cil_box* box);
 void or_destroy(or_ws* w);
 void or_set_rng(or_ws* w, int mode, uint32_t dsfmt_seed);
+/* One draw from the workspace's sequential dSFMT stream (OR_RNG_DSFMT mode): kind 0 =
+ * Random::uniform0x(1), kind 1 = Random::normal() (Random.hpp:13-52).  For host-side setup
+ * draws the reference makes on the same stream, such as initializeCellStatus. */
+double or_rng_draw(or_ws* w, int kind);
 int or_set_external_noise(or_ws* w, const double* normals, size_t per_step, int nsteps);
 /* Pair-traversal back end for inter forces: 0 = COM link list (reference structure,
  * CIL.cu:177-231), 1 = all pairs (CIL.cu:232-263). */

@@ -32,6 +32,8 @@ def lib(fast: bool = False):
     L.or_create.restype = vp
     L.or_destroy.argtypes = [vp]
     L.or_set_rng.argtypes = [vp, C.c_int, C.c_uint32]
+    L.or_rng_draw.argtypes = [vp, C.c_int]
+    L.or_rng_draw.restype = C.c_double
     L.or_set_external_noise.argtypes = [vp, abi.PD, C.c_size_t, C.c_int]
     L.or_set_traversal.argtypes = [vp, C.c_int]
     L.or_set_threads.argtypes = [vp, C.c_int]
@@ -96,6 +98,10 @@ class Oracle:
 
     def set_rng(self, mode, seed=0):
         self.L.or_set_rng(self.w, mode, seed)
+
+    def rng_draw(self, kind):
+        """One draw from the sequential dSFMT stream: 0 uniform in [0, 1), 1 normal."""
+        return self.L.or_rng_draw(self.w, kind)
 
     def set_external_noise(self, normals):
         normals = np.ascontiguousarray(normals, dtype=np.float64)

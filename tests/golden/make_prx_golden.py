@@ -1,4 +1,4 @@
-"""Write prx_seed1_frames.json: the first two snapshots of the reference's seeded single-disc
+"""Write prx_seed1_frames.json: the first seven snapshots of the reference's seeded single-disc
 HZ run (output/z0.001t3.16_v2.h5, config output/RunParams/z0.001t3.16_v2.RunParams.json,
 seed 1) and the run parameters the oracle check needs.
 
@@ -29,7 +29,7 @@ def dataset(path):
 
 def main():
     frames = {}
-    for t in (0, 1):
+    for t in range(7):
         frames[f"t_{t}"] = {k: dataset(f"/snapshots/t_{t}/{k}")
                             for k in ("R", "SigmaCore", "Theta", "RHO", "State", "T0", "PR")}
     rp = json.load(open(RUNPARAMS))
@@ -56,7 +56,7 @@ def main():
         "v1": div["V1"],
         "init": rp["INIT"]["C1"]["JSON"],
     }
-    json.dump({"source": "output/z0.001t3.16_v2.h5 (snapshots t_0, t_1) and its RunParams.json",
+    json.dump({"source": "output/z0.001t3.16_v2.h5 (snapshots t_0 to t_6) and its RunParams.json",
                "params": params, "frames": frames}, open(OUT, "w"), indent=1)
     print("wrote", OUT)
 
