@@ -160,6 +160,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-list", action="store_true", help="stencil pair kernel instead of the neighbour list")
     ap.add_argument("--skin", type=float, default=0.3)
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="steady-state repeats of at least one second each (median reported beside value)")
     ap.add_argument("--list-max", type=int, default=160)
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--train-steps", type=int, default=20000)
@@ -219,6 +221,19 @@ def main():
     md.num_cells()
     graph_elapsed = time.perf_counter() - t0
     log(f"graph path: {graph_elapsed * 1e3 / args.steps:.3f} ms/step")
+    # steady state: the median of repeats of at least one second each, graph-captured steps
+    steady = []
+    if args.repeats > 0:
+        per = max(args.steps, int(1.0 / max(graph_elapsed / args.steps, 1e-9)) + 1)
+        for _ in range(args.repeats):
+            md.num_cells()
+            t0 = time.perf_counter()
+            ts = md.step(per, ts)
+            md.num_cells()
+            steady.append((per, time.perf_counter() - t0))
+        med = sorted(e / k for k, e in steady)[len(steady) // 2]
+        log(f"steady state: median {med * 1e3:.4f} ms/step over {len(steady)} repeats of {per} steps "
+            f"({steady[0][1]:.2f} s each)")
     n = args.n
     value = n * args.steps / elapsed
     pair_us = fam.get("pair", 0.0)
@@ -252,6 +267,12 @@ def main():
         "timed_region": "steps launched directly (64-step chunks) with HIP events around the pair kernel; "
                         "kernels_us other than pair from a separate 128-step pass timing every family",
         "graph_path": {"ms_per_step": graph_elapsed * 1e3 / args.steps, "value": n * args.steps / graph_elapsed},
+        "steady_state": None if not steady else {
+            "repeats": len(steady), "steps_each": steady[0][0],
+            "seconds_each": [round(e, 3) for _, e in steady],
+            "median_ms_per_step": sorted(e / k for k, e in steady)[len(steady) // 2] * 1e3,
+            "median_value": n / sorted(e / k for k, e in steady)[len(steady) // 2],
+            "path": "graph-captured steps, no events"},
     }
     if not args.no_surrogate:
         out["surrogate"] = surrogate_leg(md, ts, n, value, args, p, box, col)
