@@ -162,7 +162,7 @@ def main():
     ap.add_argument("--skin", type=float, default=0.3)
     ap.add_argument("--repeats", type=int, default=5,
                     help="steady-state repeats of at least one second each (median reported beside value)")
-    ap.add_argument("--list-max", type=int, default=160)
+    ap.add_argument("--list-max", type=int, default=256)
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--judge-every", type=int, default=16)

@@ -198,7 +198,7 @@ class HipMD:
     def set_kernel(self, variant):
         _check(self.L, self.ctx, self.L.cil_hip_set_kernel(self.ctx, int(variant)), "set_kernel")
 
-    def set_neighbour_list(self, on=True, skin=0.3, max_neighbours=160):
+    def set_neighbour_list(self, on=True, skin=0.3, max_neighbours=256):
         _check(self.L, self.ctx, self.L.cil_hip_set_neighbour_list(self.ctx, 1 if on else 0, float(skin),
                                                                    int(max_neighbours)), "set_neighbour_list")
 

@@ -149,7 +149,7 @@ class Session {
     run_assert(cil_hip_set_tuning(ctx_, bins_per_cutoff, block), ctx_, "cil_hip_set_tuning");
   }
   // Neighbour-list mode for FP32 forces (project addition; cil_hip_set_neighbour_list)
-  void setNeighbourList(bool on, double skin = 0.3, int max_neighbours = 160) {
+  void setNeighbourList(bool on, double skin = 0.3, int max_neighbours = 256) {
     run_assert(cil_hip_set_neighbour_list(ctx_, on ? 1 : 0, skin, max_neighbours), ctx_,
                "cil_hip_set_neighbour_list");
   }
