@@ -26,7 +26,9 @@ NOISE_NONE, NOISE_CELL_WIENER, NOISE_FDT = 0, 1, 2
 # cil_rng
 RNG_COUNTER, RNG_EXTERNAL = 0, 1
 # cil_division
-DIV_NONE, DIV_THETA_CYCLE, DIV_LINEAR_GROWTH = 0, 1, 2
+DIV_NONE, DIV_THETA_CYCLE, DIV_LINEAR_GROWTH, DIV_BINARY = 0, 1, 2, 3
+# cil_integrator
+INT_EULER, INT_HEUN = 0, 1
 # cil_precision
 PREC_FP64, PREC_FP32 = 0, 1
 # CellState (Cell.cuh:26-30)
@@ -60,6 +62,9 @@ class Species(C.Structure):
         ("cycle_p_shift_tau_inv", C.c_double), ("apop_pressure", C.c_double),
         ("division_tau", Gaussian), ("migration_tau", Gaussian),
         ("growth_rate", C.c_double), ("l_div", C.c_double), ("l_birth", C.c_double), ("sigma_jitter", C.c_double),
+        ("no_division", C.c_int), ("division_swelling", C.c_int),
+        ("division_r0", C.c_double), ("division_r2min", C.c_double), ("division_r2min_ratio", C.c_double),
+        ("growthk0", D2), ("growthk1", D2), ("growthk2", D2),
     ]
 
 
@@ -83,6 +88,7 @@ class Params(C.Structure):
         ("nwalls", C.c_int), ("walls", Wall * MAX_WALLS),
         ("drain", C.c_int), ("precision", C.c_int), ("pressure", C.c_int),
         ("max_disp", C.c_double),
+        ("integrator", C.c_int),
     ]
 
 

@@ -12,6 +12,10 @@ extension is named in include/cil_hip.h.
   bulk_3d_mixed    3D periodic bulk of single discs, lognormal diameters at packing fraction
                    0.6, LJ (2.5 sigma) between species-0 pairs and soft core otherwise
   theta_2d_hz      single-disc Hertz colony with the reference's theta cell cycle
+  binary_2d_fene   two-disc cells with the reference's FENE / push-spring bond, soft-core
+                   contacts, per-cell Wiener noise and the reference's two-disc division cycle
+                   (calculDivisions: Crawling <-> Dividing, growth activity, swelling), for the
+                   Euler or the stochastic Heun integrator
 """
 from __future__ import annotations
 
@@ -307,4 +311,55 @@ def theta_2d_hz(n_cells=16, L=40.0, seed=3, dt=1e-3, capacity=256, precision=abi
     s["rho"][:n_cells] = 1.0
     s["theta"][:n_cells] = rng.uniform(0, 1, n_cells)
     s["ang_vel"][:n_cells] = 1.0 / np.maximum(np.rint(rng.normal(0.02, 0.002, n_cells) / dt), 1)
+    return p, box, col
+
+
+# ------------------------------------------------------------------------------------------
+# two-disc division cycle (calculDivisions, CIL.cu:369-455) with the reference's FENE bond
+# ------------------------------------------------------------------------------------------
+def binary_2d_fene(n_cells=150, L=20.0, seed=9, dt=2e-3, capacity=1024, integrator=abi.INT_EULER,
+                   swelling=1, D=0.005, precision=abi.PREC_FP64):
+    """Parameter names follow Input.cpp:117-245: FENE KAPPA / KAPPA_PUSH / SPRING_RATIO / RMAX
+    (r2max = RMAX^2) and RMAX_DIVISION, DIVISION R0 / Rmin / TAU_DIVISION / TAU_MIGRATION /
+    SWELLING, GROWTHK0..2 (the Dividing-state activity).  Crawling cells pull at their front disc
+    (rss-scaled motility); a Dividing cell's growth activity (growthk1 > kappa) stretches its
+    bond until |r_1 - r_0| >= Rmin and it divides into two cells of discs +-R0 apart."""
+    rng = np.random.default_rng(seed)
+    p = base_params(2, 1, 2)
+    sp = p.species[0]
+    sp.sigma[0], sp.sigma[1] = 0.9, 1.0
+    sp.spring = abi.SPRING_FENE
+    sp.kappa, sp.kappa_push, sp.spring_ratio = 1.0, 5.0, 0.3
+    sp.r2max, sp.div_r2max = 2.0, 2.5
+    sp.rss = 0.6
+    sp.motility[0], sp.motility[1] = 0.02, 0.05
+    for e in range(2):
+        sp.eps_core[e] = 1.0     # SC EPSILON_CORE
+        sp.eps_well[e] = 0.0     # no attraction
+        sp.sig_well[e] = 0.3     # SIGMA_RATIO * SIGMA_CORE
+        sp.growthk1[e] = 2.0
+    sp.division_r0 = 0.3
+    sp.division_r2min = 1.2 ** 2
+    sp.division_swelling = swelling
+    sp.division_tau.mu, sp.division_tau.sigma = 1.0, 0.2
+    sp.migration_tau.mu, sp.migration_tau.sigma = 0.2, 0.05
+    sp.d_noise = math.sqrt(2 * D) / math.sqrt(dt)
+    set_law(p, abi.LAW_SC, 1.0)
+    p.dt = dt
+    p.noise = abi.NOISE_CELL_WIENER if D > 0 else abi.NOISE_NONE
+    p.seed = seed
+    p.division = abi.DIV_BINARY
+    p.integrator = integrator
+    p.precision = precision
+    box = make_box([L, L], [1, 1])
+    col = Colony(2, 2, [capacity], ncells=[n_cells])
+    s = col.species[0]
+    c = poisson_centres(n_cells, [L, L], 1.2, rng)
+    ang = rng.uniform(0, 2 * math.pi, n_cells)
+    u = np.stack([np.cos(ang), np.sin(ang)], axis=1)
+    s["pos"][:n_cells, 0] = np.mod(c - 0.3 * u, L)
+    s["pos"][:n_cells, 1] = np.mod(c + 0.3 * u, L)
+    s["sigma"][:n_cells] = [0.85, 0.95]
+    s["state"][:n_cells] = abi.CS_CRAWLING
+    s["tswitch"][:n_cells] = rng.integers(1, 150, n_cells)
     return p, box, col

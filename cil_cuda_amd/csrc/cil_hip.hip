@@ -519,13 +519,14 @@ __device__ inline PairLaw load_law(const StepParams& P, int tp, int tq) {
 }
 
 // Random direction of an exactly coincident 2D contact pair (MD.cu:321-331), canonical i.
-__device__ inline void coincident_dir(const StepParams& P, int pidx_i, int pidx_j, uint32_t ts, double* rij) {
+__device__ inline void coincident_dir(const StepParams& P, int pidx_i, int pidx_j, uint32_t ts, uint32_t stream,
+                                      double* rij) {
   double u[2];
   int ci = pidx_i / P.nE, cj = pidx_j / P.nE;
   int ei = pidx_i - ci * P.nE, ej = pidx_j - cj * P.nE;
   int si_ = species_of_cell(P, ci), sj_ = species_of_cell(P, cj);
   uint32_t ki = (cell_key(si_, ci - P.base[si_]) << 1) | ei, kj = (cell_key(sj_, cj - P.base[sj_]) << 1) | ej;
-  philox_uniform2(P.seed, ki, ts, ST_PAIR, kj, u);
+  philox_uniform2(P.seed, ki, ts, stream, kj, u);
   double alpha = u[0] * kTwoPi;
   rij[0] = sin(alpha);
   rij[1] = cos(alpha);
@@ -534,7 +535,8 @@ __device__ inline void coincident_dir(const StepParams& P, int pidx_i, int pidx_
 // FP64 pair evaluation from p's side with the reference's operation order.
 // d = minimg(x_q - x_p) exactly as BoundingBox::distance; adds p's share to F and pr.
 __device__ inline void pair64(const StepParams& P, const PairLaw& E, int pidx_p, int spec_p, double sp_, int pidx_q,
-                              int spec_q, double sq_, double d[3], double r2, uint32_t ts, double F[3], double* pr) {
+                              int spec_q, double sq_, double d[3], double r2, uint32_t ts, uint32_t st_pair, double F[3],
+                              double* pr) {
   const int D = P.box.dim;
   const bool p_is_i = pidx_p < pidx_q;
   double si = p_is_i ? sp_ : sq_, sj = p_is_i ? sq_ : sp_;
@@ -559,7 +561,7 @@ __device__ inline void pair64(const StepParams& P, const PairLaw& E, int pidx_p,
     contact = true;
     if (D == 2 && d[0] == 0.0 && d[1] == 0.0) {
       double rij[2];
-      coincident_dir(P, p_is_i ? pidx_p : pidx_q, p_is_i ? pidx_q : pidx_p, ts, rij);
+      coincident_dir(P, p_is_i ? pidx_p : pidx_q, p_is_i ? pidx_q : pidx_p, ts, st_pair, rij);
       d[0] = p_is_i ? rij[0] : -rij[0];
       d[1] = p_is_i ? rij[1] : -rij[1];
     }
@@ -635,7 +637,7 @@ __device__ inline void pair32(const StepParams& P, const Dev& S, const PairLaw& 
     contact = true;
     if (D == 2 && d[0] == 0.0f && d[1] == 0.0f) {
       double rij[2];
-      coincident_dir(P, p_is_i ? pidx_p : pidx_q, p_is_i ? pidx_q : pidx_p, ts, rij);
+      coincident_dir(P, p_is_i ? pidx_p : pidx_q, p_is_i ? pidx_q : pidx_p, ts, force_stream(S, ST_PAIR), rij);
       d[0] = (float)(p_is_i ? rij[0] : -rij[0]);
       d[1] = (float)(p_is_i ? rij[1] : -rij[1]);
     }
@@ -842,8 +844,8 @@ __device__ inline void k_pair_body(const StepParams* __restrict__ Pp, const Dev&
               if (r2 > rc * rc * margin || (qidx >> cshift) == cell_p) continue;
               double Fl[3] = {(double)F[0], (double)F[1], (double)F[2]};
               double prl = pr;
-              pair64(P, E, pidx, spec_p, (double)xp[3], qidx, species_of_cell(P, qidx >> cshift), r.w, d, r2, ts, Fl,
-                     &prl);
+              pair64(P, E, pidx, spec_p, (double)xp[3], qidx, species_of_cell(P, qidx >> cshift), r.w, d, r2, ts,
+                     force_stream(S, ST_PAIR), Fl, &prl);
               F[0] = Fl[0]; F[1] = Fl[1]; F[2] = Fl[2];
               pr = prl;
             }
@@ -953,7 +955,7 @@ __device__ __forceinline__ bool pair_term32(const StepParams& P, const Dev& S, c
     contact = true;
     if (D == 2 && dx == 0.0f && dy == 0.0f) {
       double rij[2];
-      coincident_dir(P, p_is_i ? pidx : qidx, p_is_i ? qidx : pidx, ts, rij);
+      coincident_dir(P, p_is_i ? pidx : qidx, p_is_i ? qidx : pidx, ts, force_stream(S, ST_PAIR), rij);
       dx = (float)(p_is_i ? rij[0] : -rij[0]);
       dy = (float)(p_is_i ? rij[1] : -rij[1]);
     }
@@ -1221,6 +1223,7 @@ __device__ inline void build_list_of(const StepParams& P, const Dev& S, int tp, 
   const int nE = P.nE, cshift = nE - 1, spec_p = tp / nE;
   const int pidx = S.smeta[k] >> 3, cell_p = pidx >> cshift;
   for (int d = 0; d < D; ++d) S.xref[(size_t)d * S.P + pidx] = (float)S.pos[(size_t)d * S.P + pidx];
+  S.xref[(size_t)3 * S.P + pidx] = (float)S.sigma[pidx];
   int bc[3];
   unpack_bin(P, S.sbin[k], bc);
   const float4 me = spos[k];
@@ -1507,6 +1510,9 @@ __device__ inline void k_pair_list_body(const StepParams* __restrict__ Pp, const
   __shared__ float4 bwin[LIST_BLOCK / WAVE][BUILD_WIN];  // per-wave candidate windows of the build
   list_offsets_fill(P, tab);
   const bool rebuild = *rebuild_flag(S, step_ts(S)) != 0u;
+  // a re-sort whose grid barrier timed out left the sorted arrays half built: no list is built
+  // from them (the host reports the error at its next check)
+  if (rebuild && (__hip_atomic_load(S.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_SYNC)) return;
   int tp, k;
   if (!wave_type_slot(P, S, &tp, &k)) return;
   if (rebuild) build_list_of<D>(P, S, tp, k, bwin[threadIdx.x / WAVE]);
@@ -1615,7 +1621,7 @@ __device__ inline void k_noise_body(const StepParams* __restrict__ Pp, const Dev
         if (blk) z = blk[at];
         else {
           double zz[2];
-          philox_normal2(P.seed, cell_key(s, c), (uint32_t)ts, ST_WIENER, (uint32_t)(d >> 1), zz);
+          philox_normal2(P.seed, cell_key(s, c), (uint32_t)ts, force_stream(S, ST_WIENER), (uint32_t)(d >> 1), zz);
           z = zz[d & 1];
         }
         w = sp.d_noise * z;
@@ -1640,7 +1646,7 @@ __device__ inline void k_noise_body(const StepParams* __restrict__ Pp, const Dev
       atomicOr(S.err, (unsigned)ERR_NOISE);
       return;
     }
-    if (!blk) philox_normal2(P.seed, cell_key(s, c), (uint32_t)ts, ST_ACTIVE, 0, zz);
+    if (!blk) philox_normal2(P.seed, cell_key(s, c), (uint32_t)ts, force_stream(S, ST_ACTIVE), 0, zz);
     double wr = 0, wa = 0;
     if (sp.d_noise_radial > 0.0) wr = sp.d_noise_radial * (blk ? blk[ar + c] : zz[0]);
     if (sp.d_noise_angular > 0.0) wa = sp.d_noise_angular * (blk ? blk[aa + c] : zz[1]);
@@ -1716,11 +1722,16 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
   double pND = pc / sp.cycle_p0;
   for (int e = 0; e < nE; ++e) {
     size_t p = (size_t)cslot * nE + e;
-    double pterm = sp.zeta_p[e] == 0.0 ? 0.0 : sp.zeta_p[e] * pow(pND, sp.zeta_p_beta[e]);
-    double temp = P.dt * 1 / ((sp.zeta[e] + pterm) * area);
+    double temp;
+    if (S.heun) {  // updatePositions with the half step (CIL.cu:849-866; hdt = 0.5 dt, CIL.cuh:208)
+      temp = (0.5 * P.dt) * (1.0 / sp.zeta[e]);
+    } else {
+      double pterm = sp.zeta_p[e] == 0.0 ? 0.0 : sp.zeta_p[e] * pow(pND, sp.zeta_p_beta[e]);
+      temp = P.dt * 1 / ((sp.zeta[e] + pterm) * area);
+    }
     double mu = temp / P.dt;
     double z[4] = {0, 0, 0, 0};
-    bool fdt = P.noise == CIL_NOISE_FDT && sp.kT > 0.0;
+    bool fdt = !S.heun && P.noise == CIL_NOISE_FDT && sp.kT > 0.0;
     if (fdt) {
       uint32_t id = (cell_key(s, c) << 1) | (uint32_t)e;
       if (sizeof(T) == 4) {  // FP32 force mode: FP32 Box-Muller on the same draws
@@ -1733,7 +1744,7 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
     }
     double amp = fdt ? sqrt(2.0 * sp.kT * temp) : 0.0;
     double xn[3];
-    if (fdt || P.max_disp > 0.0) {
+    if (fdt || (!S.heun && P.max_disp > 0.0)) {
       double dx[3] = {0, 0, 0}, n2 = 0.0;
       for (int d = 0; d < D; ++d) {
         dx[d] = fdt ? temp * F[e][d] + amp * z[d] : temp * F[e][d];
@@ -1749,7 +1760,10 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
     }
     for (int d = 0; d < D; ++d) {
       S.pos[(size_t)d * S.P + p] = xn[d];
-      S.vel[(size_t)d * S.P + p] = F[e][d] * mu;
+      // Heun leaves the velocities alone, except for the zero-velocity kick of a single disc
+      // (MD.cu:609-614), which the reference writes into the velocity array
+      if (!S.heun) S.vel[(size_t)d * S.P + p] = F[e][d] * mu;
+      else if (nE == 1 && need_vel) S.vel[(size_t)d * S.P + p] = vel[e][d];
     }
     if (emit_keys == 1) {
       bool finite = true;
@@ -1775,7 +1789,7 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
         dr2 = fmaf(dd, dd, dr2);
       }
       if (!finite) atomicOr(S.err, (unsigned)ERR_NONFINITE);
-      unsigned* next = rebuild_flag(S, tsu + 1);  // flag of this step's force pass
+      unsigned* next = next_pass_flag(S, tsu);  // flag of the force pass that follows
       const bool moved = !(dr2 <= 0.25f * P.skin * P.skin);
       // one atomic per wave (its first moved lane), none once the flag is up
       const unsigned long long mv = __ballot(moved);

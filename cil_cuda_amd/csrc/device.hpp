@@ -17,6 +17,10 @@ constexpr double kTolMp = 10.0 * 2.220446049250313e-16;
 constexpr double kWallCutoff = 0.82;  // Wall.cpp:3
 
 enum Stream : uint32_t { ST_WIENER = 1, ST_FDT = 2, ST_ACTIVE = 3, ST_DIV = 4, ST_ZERO = 5, ST_PAIR = 6 };
+// The stochastic Heun step evaluates forces twice per step: the corrector's evaluation (at the
+// half-step positions, before the step index moves on) adds ST_PHASE to the streams of its
+// noise and zero-distance draws, so it never repeats the previous step's final evaluation.
+constexpr uint32_t ST_PHASE = 8;
 enum CellStateV : int { CS_DIVIDING = 0, CS_CRAWLING = 1, CS_UNDEFINED = 2 };  // Cell.cuh:26-30
 
 // Error bits raised by kernels and collected by the host at synchronisation points.
@@ -178,6 +182,8 @@ struct StepParams {
   int list_mode;
   int list_max;     // list capacity per particle
   float skin;
+  float list_rca_max;  // largest rca of any type pair with a law: a pair cut-off grows by at most
+                       // list_rca_max per unit of diameter growth of either particle
   cil_wall walls[CIL_MAX_WALLS];
   cil_species sp[CIL_MAX_SPECIES];
   // bins: spatial grid ns[0] x ns[1] x ns[2]; particles are counting-sorted into buckets

@@ -71,7 +71,12 @@ struct Dev {
   int* nl;            // [list_max / 4][P][4] groups of (sorted index << 10 | bin offset code);
                       // each source type starts on a group boundary
   int* nlt;           // [ntypes][P] entries per source type
-  float* xref;        // [3][P] positions at the last build (particle order)
+  float* xref;        // [4][P] positions (x, y, z) and diameter at the last build (particle order)
+  // integrator: 0 Euler (one force pass per step, at step index f); 1 stochastic Heun (two
+  // passes per step: the corrector's at the step's own index with phase 1, the final one at the
+  // next index with phase 0).  `phase` is set per launch.
+  int heun;
+  int phase;
 };
 
 // ---------------------------------------------------------------------------------------
@@ -80,8 +85,17 @@ struct Dev {
 // Step index seen by a kernel: the device counter plus the launch's offset within its chunk
 // (a chunk of steps advances the counter once, at its end).
 __device__ inline int step_ts(const Dev& S) { return *S.ts + S.ts_off; }
-// Rebuild flag of the force pass at step index f (list mode).
-__device__ inline unsigned* rebuild_flag(const Dev& S, int f) { return S.rebuild + (f & 1); }
+// Rebuild flag of the force pass at step index f (list mode).  Consecutive force passes use
+// alternate words: by the step index's parity under Euler, by the phase under Heun (corrector
+// passes word 1, final passes word 0).  A launch's own pass, and the update's previous pass,
+// is the one of its (step index, phase).
+__device__ inline unsigned* rebuild_flag(const Dev& S, int f) { return S.rebuild + (S.heun ? S.phase : (f & 1)); }
+// Flag of the force pass that follows an update launched at step index f.
+__device__ inline unsigned* next_pass_flag(const Dev& S, int f) {
+  return S.rebuild + (S.heun ? 1 - S.phase : ((f + 1) & 1));
+}
+// Streams of the draws made while forces are evaluated, offset in the Heun corrector phase.
+__device__ inline uint32_t force_stream(const Dev& S, uint32_t st) { return st + ST_PHASE * (uint32_t)S.phase; }
 
 __device__ inline bool cell_active(const StepParams& P, const int* ncells, int cslot, int* s_out, int* c_out) {
   int s = species_of_cell(P, cslot);
@@ -132,10 +146,12 @@ __device__ inline void flat_wall(const cil_wall& wl, double sigma, const double*
   }
 }
 
-__device__ inline void on_sphere_counter(uint64_t seed, uint32_t id, uint32_t ts, int D, double* n) {
+// Random unit vector from Philox blocks k0, k0 + 1 of (id, ts, stream): normalised Gaussians.
+__device__ inline void on_sphere_counter(uint64_t seed, uint32_t id, uint32_t ts, int D, double* n,
+                                         uint32_t stream = ST_ZERO, uint32_t k0 = 0) {
   double z[4];
-  philox_normal2(seed, id, ts, ST_ZERO, 0, z);
-  philox_normal2(seed, id, ts, ST_ZERO, 1, z + 2);
+  philox_normal2(seed, id, ts, stream, k0, z);
+  philox_normal2(seed, id, ts, stream, k0 + 1, z + 2);
   double r2 = 0.0;
   for (int d = 0; d < D; ++d) {
     n[d] = z[d];
@@ -160,30 +176,44 @@ __device__ inline void intra_cell(const StepParams& P, const Dev& S, int s, int 
     double r = sqrt(r2);
     if (r2 == 0.0) {  // MD.cu:534-540
       r = kBigTolMp * 0.5 * (sp.sigma[0] + sp.sigma[1]);
-      on_sphere_counter(P.seed, cell_key(s, c), (uint32_t)ts, D, rij);
+      on_sphere_counter(P.seed, cell_key(s, c), (uint32_t)ts, D, rij, force_stream(S, ST_ZERO));
       for (int d = 0; d < D; ++d) rij[d] *= r;
       r2 = sq(r);
       for (int d = 0; d < D; ++d) pos[1][d] = box_update_dr1(P.box, d, pos[1][d], rij[d]);
       *pos1_changed = true;
     }
     for (int d = 0; d < 3; ++d) nij[d] = rij[d];
-    double spring, scf;
+    double spring, scf, a_div[2] = {0.0, 0.0};
+    bool use_div = false;
     if (sp.spring == CIL_SPRING_HARMONIC) {
       double l = S.rest_len[cslot];
       spring = -sp.k_harm * (r - l) / r;
       scf = 1.0;
       for (int d = 0; d < D; ++d) nij[d] = rij[d] / r;
     } else {
+      // Dividing state (MD.cu:547,560-568): divisionR2max replaces r2max, the activity of both
+      // discs becomes -growthk0 fene(divisionR2min) + growthk1 + growthk2 r2^2 along the
+      // unscaled axis, and the contraction factor 2 becomes 1
+      const bool dividing = S.state[cslot] == CS_DIVIDING;
       double smax = sig[0] >= sig[1] ? sig[0] : sig[1];
-      double r2max = sp.r2max > smax ? sp.r2max : smax;
+      double r2max0 = dividing ? sp.div_r2max : sp.r2max;
+      double r2max = r2max0 > smax ? r2max0 : smax;
       double ratio = sp.spring_ratio;
       if (!(r2max - r2 > kTolMp)) atomicOr(S.err, (unsigned)ERR_BOND);
       spring = r2 <= (r2max * ratio) ? sp.kappa_push * fabs(r2 - (r2max * ratio))
                                      : -sp.kappa / (1.0 - sq((r2 - r2max * ratio) / (r2max * (1 - ratio))));
-      scf = 2.0;
-      if (!sp.cil) {
-        double scaling = sp.rss / r;
-        for (int d = 0; d < D; ++d) nij[d] *= scaling;
+      if (dividing) {
+        const double fene_div_max = -sp.kappa / (1.0 - sq(sp.division_r2min / r2max));  // FENE, MD.cu:255-257
+        a_div[0] = (-sp.growthk0[0] * fene_div_max + sp.growthk1[0] + sp.growthk2[0] * sq(r2));
+        a_div[1] = (-sp.growthk0[1] * fene_div_max + sp.growthk1[1] + sp.growthk2[1] * sq(r2));
+        use_div = true;
+        scf = 1.0;
+      } else {
+        scf = 2.0;
+        if (!sp.cil) {
+          double scaling = sp.rss / r;
+          for (int d = 0; d < D; ++d) nij[d] *= scaling;
+        }
       }
     }
     if (P.active_noise && D == 2) {
@@ -191,7 +221,7 @@ __device__ inline void intra_cell(const StepParams& P, const Dev& S, int s, int 
       rnd[0] = 0.5 * (Wr * rij[0] / r - Wt * rij[1]);
       rnd[1] = 0.5 * (Wr * rij[1] / r + Wt * rij[0]);
     }
-    double a0 = sp.motility[0], a1 = sp.motility[1];
+    const double a0 = use_div ? a_div[0] : sp.motility[0], a1 = use_div ? a_div[1] : sp.motility[1];
     for (int d = 0; d < D; ++d) {
       double tc0 = -(a0 * nij[d] + spring * rij[d] * scf + rnd[d] * rij[d]);
       double tc1 = (a1 * nij[d] + spring * rij[d] * scf + rnd[d] * rij[d]);
@@ -211,7 +241,7 @@ __device__ inline void intra_cell(const StepParams& P, const Dev& S, int s, int 
     for (int d = 0; d < D; ++d) vl += vel[0][d] * vel[0][d];
     vl = sqrt(vl);
     if (vl == 0.0 && sp.motility[0] != 0.0) {  // MD.cu:609-614
-      on_sphere_counter(P.seed, cell_key(s, c) << 1, (uint32_t)ts, D, vel[0]);
+      on_sphere_counter(P.seed, cell_key(s, c) << 1, (uint32_t)ts, D, vel[0], force_stream(S, ST_ZERO));
       for (int d = 0; d < D; ++d) vel[0][d] *= kBigTolMp;
       vl = 0.0;
       for (int d = 0; d < D; ++d) vl += vel[0][d] * vel[0][d];

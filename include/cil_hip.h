@@ -72,8 +72,17 @@ typedef enum {
 typedef enum {
   CIL_DIV_NONE = 0,
   CIL_DIV_THETA_CYCLE = 1,   /* calculDivisionsSingle, CIL.cu:469-619 (single-disc cells) */
-  CIL_DIV_LINEAR_GROWTH = 2  /* extension: two-disc rest length grows at rate 1/tau, divide at l_div */
+  CIL_DIV_LINEAR_GROWTH = 2, /* extension: two-disc rest length grows at rate 1/tau, divide at l_div */
+  CIL_DIV_BINARY = 3         /* calculDivisions, CIL.cu:369-455 (two-disc cells: Crawling <-> Dividing,
+                                division check binaryCellDivisionCriteria MD.cuh:426-432, geometry
+                                binaryCellDivision / binaryCellDivisionFail MD.cu:721-752) */
 } cil_division;
+
+typedef enum {
+  CIL_INT_EULER = 0, /* Euler(int&), CIL.cu:1013-1020 */
+  CIL_INT_HEUN = 1   /* StochasticHeun(int&), CIL.cu:1028-1049: half step with the old forces, forces at the
+                        half-step positions, second half step, divisions, forces */
+} cil_integrator;
 
 typedef enum {
   CIL_PREC_FP64 = 0, /* forces in FP64 (parity mode) */
@@ -124,6 +133,16 @@ typedef struct {
   double l_div;                     /* divide when the rest length reaches l_div */
   double l_birth;                   /* daughters' rest length */
   double sigma_jitter;              /* daughter diameters sigma*(1 + jitter*N(0,1)) */
+  /* division switches and the two-disc division cycle (Input.cpp:148-245) */
+  int no_division;                  /* CellParams::DIV == false: the species never divides (its divisionFunc
+                                       is NULL, CIL.cuh:18-47); the default 0 divides under `division` */
+  int division_swelling;            /* DIVISION.SWELLING: updateDiameters (CIL.cu:970-990) */
+  double division_r0;               /* divisionR0: daughter half-separation (MD.cu:721-752) */
+  double division_r2min;            /* divisionR2min = Rmin^2: division check and feneDivMax (MD.cu:560) */
+  double division_r2min_ratio;      /* divisionR2minRatio: when nonzero the check is r2 >= ratio*sigma0 */
+  double growthk0[CIL_MAX_ELEMS];   /* GROWTHK0..2: Dividing-state activity a_e = -k0 fene(Rmin^2) + k1 + k2 r^4 */
+  double growthk1[CIL_MAX_ELEMS];   /*   (binaryCellForce, MD.cu:561-566) */
+  double growthk2[CIL_MAX_ELEMS];
 } cil_species;
 
 typedef struct {
@@ -158,6 +177,7 @@ typedef struct {
   int pressure;                     /* accumulate pair pressure (needed when zeta_p != 0 or theta cycle) */
   double max_disp;                  /* extension for relaxing random initial states: per-step displacement
                                        cap (0 = off, the reference behaviour) */
+  int integrator;                   /* cil_integrator (MDParams::INTEGRATOR, Input.cpp:520-540) */
 } cil_params;
 
 typedef struct {

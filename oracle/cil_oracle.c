@@ -34,6 +34,11 @@
 #define WALL_CUTOFF 0.82 /* Wall.cpp:3 */
 
 enum { ST_WIENER = 1, ST_FDT = 2, ST_ACTIVE = 3, ST_DIV = 4, ST_ZERO = 5, ST_PAIR = 6 };
+/* The stochastic Heun step evaluates forces twice per step: the corrector's evaluation (at the
+ * half-step positions, before ++ts) draws its noise and its zero-distance directions from the
+ * streams above with ST_PHASE added, so that it never repeats the draws of the previous step's
+ * final evaluation at the same step index. */
+enum { ST_PHASE = 8 };
 enum { CS_DIVIDING = 0, CS_CRAWLING = 1, CS_UNDEFINED = 2 }; /* Cell.cuh:26-30 */
 
 /* ===================================================================================== */
@@ -277,6 +282,7 @@ struct or_ws {
   double width;
   double* com;
   int com_cap;
+  int phase; /* 1 while the Heun corrector's forces are evaluated (stream offset ST_PHASE) */
   char err[256];
 };
 
@@ -379,7 +385,12 @@ static double rng_normal(or_ws* w, uint32_t id, uint32_t ts, uint32_t stream, ui
 
 /* Random direction on the D-sphere.  dSFMT source: Random::onDSphere (Random.hpp:82-114,
  * rejection in the unit ball with Random::uniform()).  Counter source: normalised Gaussian. */
+static void rng_on_sphere_k(or_ws* w, double* n, uint32_t id, uint32_t ts, uint32_t stream, uint32_t k0);
 static void rng_on_sphere(or_ws* w, double* n, uint32_t id, uint32_t ts, uint32_t stream) {
+  rng_on_sphere_k(w, n, id, ts, stream, 0);
+}
+/* counter source: Philox blocks k0 and k0 + 1 of (id, ts, stream) */
+static void rng_on_sphere_k(or_ws* w, double* n, uint32_t id, uint32_t ts, uint32_t stream, uint32_t k0) {
   int d, D = w->box.dim;
   double r2 = 0.0, norm;
   if (w->rng_mode == OR_RNG_DSFMT) {
@@ -392,8 +403,8 @@ static void rng_on_sphere(or_ws* w, double* n, uint32_t id, uint32_t ts, uint32_
     } while (!(r2 <= 1.0));
   } else {
     double z[4];
-    or_philox_normal2(w->p.seed, id, ts, stream, 0, z);
-    or_philox_normal2(w->p.seed, id, ts, stream, 1, z + 2);
+    or_philox_normal2(w->p.seed, id, ts, stream, k0, z);
+    or_philox_normal2(w->p.seed, id, ts, stream, k0 + 1, z + 2);
     for (d = 0; d < D; d++) {
       n[d] = z[d];
       r2 += n[d] * n[d];
@@ -537,7 +548,7 @@ static void elem_pair(or_ws* w, int ti, int tj, int spi, int spj, const double* 
     double A, temp;
     if (!(r <= (si + sj) / 2)) return;
     if (D == 2 && rij[0] == 0 && rij[1] == 0) { /* MD.cu:321-331 */
-      double alpha = rng_uniform0x(w, OR_TWOPI, pid_i, (uint32_t)ts, ST_PAIR, pid_j, 0);
+      double alpha = rng_uniform0x(w, OR_TWOPI, pid_i, (uint32_t)ts, ST_PAIR + ST_PHASE * w->phase, pid_j, 0);
       rij[0] = sin(alpha);
       rij[1] = cos(alpha);
     }
@@ -821,7 +832,7 @@ static void binary_cell_force(or_ws* w, cil_species_view* v, int s, int c, int t
   double act0 = P->motility[0], act1 = P->motility[1];
   if (r2 == 0.0) { /* MD.cu:534-540 */
     r = OR_BIG_TOL_MP * 0.5 * (P->sigma[0] + P->sigma[1]);
-    rng_on_sphere(w, rij, cell_key(s, c), (uint32_t)ts, ST_ZERO);
+    rng_on_sphere(w, rij, cell_key(s, c), (uint32_t)ts, ST_ZERO + ST_PHASE * w->phase);
     for (d = 0; d < D; d++) rij[d] *= r;
     r2 = SQ(r);
     box_update_pos_dr(&w->box, p1, rij);
@@ -833,16 +844,27 @@ static void binary_cell_force(or_ws* w, cil_species_view* v, int s, int c, int t
     sc = 1.0;
     for (d = 0; d < D; d++) nij[d] = rij[d] / r;
   } else {
-    double r2max0 = P->r2max, smax = sig[0] >= sig[1] ? sig[0] : sig[1];
+    /* the Dividing state (MD.cu:547,560-568): the bond's r2max becomes divisionR2max, both
+     * discs' activity becomes -growthk0 fene(divisionR2min) + growthk1 + growthk2 r2^2 along
+     * the unscaled axis, and the contraction factor drops from 2 to 1 */
+    const int dividing = v[s].state && v[s].state[c] == CS_DIVIDING;
+    double r2max0 = dividing ? P->div_r2max : P->r2max, smax = sig[0] >= sig[1] ? sig[0] : sig[1];
     double r2max = r2max0 > smax ? r2max0 : smax;
     double ratio = P->spring_ratio;
     if (!(r2max - r2 > OR_TOL_MP)) snprintf(w->err, sizeof w->err, "FENE bond over-stretched");
     spring = r2 <= (r2max * ratio) ? P->kappa_push * fabs(r2 - (r2max * ratio))
                                    : -P->kappa / (1.0 - SQ((r2 - r2max * ratio) / (r2max * (1 - ratio))));
-    sc = 2.0;
-    if (!P->cil) {
-      double scaling = P->rss / r;
-      for (d = 0; d < D; d++) nij[d] *= scaling;
+    if (dividing) {
+      double fene_div_max = -P->kappa / (1.0 - SQ(P->division_r2min / r2max)); /* FENE, MD.cu:255-257 */
+      act0 = (-P->growthk0[0] * fene_div_max + P->growthk1[0] + P->growthk2[0] * SQ(r2));
+      act1 = (-P->growthk0[1] * fene_div_max + P->growthk1[1] + P->growthk2[1] * SQ(r2));
+      sc = 1.0;
+    } else {
+      sc = 2.0;
+      if (!P->cil) {
+        double scaling = P->rss / r;
+        for (d = 0; d < D; d++) nij[d] *= scaling;
+      }
     }
   }
   if (w->p.active_noise && D == 2) { /* MD.cu:573-578 */
@@ -876,7 +898,7 @@ static void single_cell_force(or_ws* w, cil_species_view* v, int s, int c, int t
   vl = sqrt(vl);
   if (vl == 0.0) {
     if (w->rng_mode == OR_RNG_DSFMT || P->motility[0] != 0.0) {
-      rng_on_sphere(w, v0, cell_key(s, c) << 1, (uint32_t)ts, ST_ZERO);
+      rng_on_sphere(w, v0, cell_key(s, c) << 1, (uint32_t)ts, ST_ZERO + ST_PHASE * w->phase);
       for (d = 0; d < D; d++) v0[d] *= OR_BIG_TOL_MP;
     }
     vl = 0.0;
@@ -941,7 +963,7 @@ static int draw_noise(or_ws* w, cil_species_view* v, int nspec, int ts) {
       if (!(P->d_noise > 0.0)) continue;
       for (c = 0; c < v[s].ncells; c++)
         for (d = 0; d < D; d++) {
-          double z = ext ? ext[pos++] : rng_normal(w, cell_key(s, c), (uint32_t)ts, ST_WIENER, (uint32_t)(d >> 1), d & 1);
+          double z = ext ? ext[pos++] : rng_normal(w, cell_key(s, c), (uint32_t)ts, ST_WIENER + ST_PHASE * w->phase, (uint32_t)(d >> 1), d & 1);
           w->wiener[s][(size_t)c * D + d] = P->d_noise * z;
         }
     }
@@ -952,11 +974,11 @@ static int draw_noise(or_ws* w, cil_species_view* v, int nspec, int ts) {
       if (P->d_noise_radial > 0.0)
         for (c = 0; c < v[s].ncells; c++)
           w->awiener[s][(size_t)c * 2] =
-              P->d_noise_radial * (ext ? ext[pos++] : rng_normal(w, cell_key(s, c), (uint32_t)ts, ST_ACTIVE, 0, 0));
+              P->d_noise_radial * (ext ? ext[pos++] : rng_normal(w, cell_key(s, c), (uint32_t)ts, ST_ACTIVE + ST_PHASE * w->phase, 0, 0));
       if (P->d_noise_angular > 0.0)
         for (c = 0; c < v[s].ncells; c++)
           w->awiener[s][(size_t)c * 2 + 1] =
-              P->d_noise_angular * (ext ? ext[pos++] : rng_normal(w, cell_key(s, c), (uint32_t)ts, ST_ACTIVE, 0, 1));
+              P->d_noise_angular * (ext ? ext[pos++] : rng_normal(w, cell_key(s, c), (uint32_t)ts, ST_ACTIVE + ST_PHASE * w->phase, 0, 1));
     }
   if (ext && pos != w->ext_per_step) return fail(w, CIL_E_INVALID, "external noise block size mismatch");
   return 0;
@@ -1156,7 +1178,7 @@ static int divisions_theta(or_ws* w, cil_species_view* v, int nspec, int ts) {
     for (i = 0; i < nCells; i++) {
       if ((V->state[i] != CS_DIVIDING) || (V->pressure[i] < P->apop_pressure)) V->tpressure[i] = 0;
       else V->tpressure[i] += 1;
-      if (V->state[i] == CS_DIVIDING) {
+      if (V->state[i] == CS_DIVIDING && !P->no_division) { /* func != NULL, CIL.cu:496 */
         double pn = V->pressure[i] + pShift;
         double pNonDim = (0. >= pn ? 0. : pn) / P->cycle_p0;
         double rr = P->cycle_rmaxed_inv * V->rho[i];
@@ -1267,7 +1289,7 @@ static int divisions_linear(or_ws* w, cil_species_view* v, int nspec, int ts) {
     const cil_species* P = &w->p.species[s];
     cil_species_view* V = &v[s];
     const int nCells = V->ncells;
-    if (!(P->growth_rate > 0.0)) continue;
+    if (!(P->growth_rate > 0.0) || P->no_division) continue;
     for (i = 0; i < nCells; i++) {
       V->rest_len[i] += P->growth_rate * w->p.dt;
       if (V->rest_len[i] >= P->l_div) {
@@ -1319,9 +1341,144 @@ static int divisions_linear(or_ws* w, cil_species_view* v, int nspec, int ts) {
   return 0;
 }
 
+/* binaryRandomReorientation (MD.cu:653-659): slave <- master, then master += radius*dr and
+ * slave -= radius*dr for a random unit direction dr (both wrapped into the box). */
+static void binary_reorient(or_ws* w, double* slave, double* master, double radius, uint32_t key, int ts,
+                            uint32_t k0) {
+  int d, D = w->box.dim;
+  double dr[3] = {0, 0, 0}, m[3];
+  for (d = 0; d < D; d++) slave[d] = master[d];
+  rng_on_sphere_k(w, dr, key, (uint32_t)ts, ST_DIV, k0);
+  for (d = 0; d < D; d++) m[d] = -dr[d];
+  box_update_pos(&w->box, master, dr, radius);
+  box_update_pos(&w->box, slave, m, radius);
+}
+
+/* resetVel / resetFrc of both discs of cell i (Utils.hpp:591-599) */
+static void reset_cell_motion(or_ws* w, cil_species_view* V, int i) {
+  int k, D = w->box.dim;
+  for (k = 0; k < 2 * D; k++) {
+    V->vel[(size_t)i * 2 * D + k] = 0;
+    V->frc[(size_t)i * 2 * D + k] = 0;
+  }
+}
+
+/* randomTimeSpan (MD.cuh:433-435): max(Nint(N(mu, sigma) / dt), 1) with the normal drawn from
+ * the dSFMT stream (Leva) or from Philox block k of (key, ts, ST_DIV). */
+static int random_span(or_ws* w, const cil_gaussian* g, uint32_t key, int ts, uint32_t k, int half) {
+  double z = rng_normal(w, key, (uint32_t)ts, ST_DIV, k, half);
+  return time_span(w->p.dt, g->mu, g->sigma, z);
+}
+
+/* calculDivisions, CIL.cu:369-455, for two-disc cells.  Per dividing species, cells i in
+ * ascending order with the count frozen at entry:
+ *   Dividing, and binaryCellDivisionCriteria (MD.cuh:426-432: |r_0 - r_1|^2 >= divisionR2min,
+ *   or >= divisionR2minRatio * sigma_0 when that ratio is set): the cell divides.  The daughter
+ *   is appended; binaryCellDivision (MD.cu:721-731) gives the daughter the mother's back disc
+ *   position and re-places both cells' discs +-divisionR0 around the mother's front and the
+ *   old back position along fresh random directions (mother first).  Both get t0 = ts and
+ *   state Crawling, vel = frc = 0 and the species' diameters; the daughter's switch time is
+ *   ts + span(tau_migration), the mother's grows by another such span.
+ *   Dividing, check failed, and tswitch == ts: binaryCellDivisionFail (MD.cu:736-752) moves
+ *   the back disc to the midpoint (unwrapped, as the reference does) and re-places both discs
+ *   +-divisionR0 around it; vel = frc = 0, tswitch += span(tau_migration), Crawling, species
+ *   diameters.
+ *   Crawling and tswitch == ts: state Dividing, tswitch += span(tau_division), vel = frc = 0.
+ * The reference draws from one sequential stream in that order (onDSphere mother, onDSphere
+ * daughter, normal daughter, normal mother; failure: onDSphere, normal; start: normal); with
+ * the counter source the draws of cell i at step ts are Philox blocks of (cell key, ts, ST_DIV):
+ * directions k = 0,1 and 2,3, the success and failure spans block 4 (daughter half 0, mother
+ * half 1), the start span block 5.  The fields the reference leaves stale in the daughter's
+ * slot (addCell, Colony.cpp:355-359: pressure, theta, rho, ang_vel, sigma_t, rest_len) take
+ * the mother's values and tpressure 0, so that runs are reproducible. */
+static int divisions_binary(or_ws* w, cil_species_view* v, int nspec, int ts) {
+  int s, i, d, e, D = w->box.dim;
+  for (s = 0; s < nspec; s++) {
+    const cil_species* P = &w->p.species[s];
+    cil_species_view* V = &v[s];
+    const int nCells = V->ncells;
+    if (P->no_division) continue;
+    for (i = 0; i < nCells; i++) {
+      const uint32_t key = cell_key(s, i);
+      double* b = &V->pos[((size_t)i * 2) * D];
+      double* f = &V->pos[((size_t)i * 2 + 1) * D];
+      if (V->state[i] == CS_DIVIDING) {
+        double rij[3];
+        double r2 = box_distance(&w->box, rij, f, b);
+        int ok = P->division_r2min_ratio != 0.0 ? r2 >= P->division_r2min_ratio * V->sigma[(size_t)i * 2]
+                                                : r2 >= P->division_r2min;
+        if (ok) {
+          int inew = add_cell(V);
+          double *nb, *nf;
+          if (inew < 0) return fail(w, CIL_E_CAPACITY, "division beyond capacity");
+          nb = &V->pos[((size_t)inew * 2) * D];
+          nf = &V->pos[((size_t)inew * 2 + 1) * D];
+          for (d = 0; d < D; d++) nb[d] = b[d];
+          binary_reorient(w, b, f, P->division_r0, key, ts, 0);   /* mother keeps the front */
+          binary_reorient(w, nf, nb, P->division_r0, key, ts, 2); /* daughter the old back */
+          V->t0[inew] = ts;
+          V->t0[i] = ts;
+          V->tswitch[inew] = ts + random_span(w, &P->migration_tau, key, ts, 4, 0);
+          V->state[inew] = CS_CRAWLING;
+          for (e = 0; e < 2; e++) V->sigma[(size_t)inew * 2 + e] = P->sigma[e];
+          reset_cell_motion(w, V, inew);
+          reset_cell_motion(w, V, i);
+          V->pressure[inew] = V->pressure[i];
+          V->tpressure[inew] = 0;
+          if (V->theta) V->theta[inew] = V->theta[i];
+          if (V->rho) V->rho[inew] = V->rho[i];
+          if (V->ang_vel) V->ang_vel[inew] = V->ang_vel[i];
+          if (V->sigma_t) V->sigma_t[inew] = V->sigma_t[i];
+          if (V->rest_len) V->rest_len[inew] = V->rest_len[i];
+          V->tswitch[i] += random_span(w, &P->migration_tau, key, ts, 4, 1);
+          V->state[i] = CS_CRAWLING;
+          for (e = 0; e < 2; e++) V->sigma[(size_t)i * 2 + e] = P->sigma[e];
+        } else if (V->tswitch[i] == ts) {
+          double dr[3];
+          box_distance(&w->box, dr, b, f); /* dr = r_1 - r_0 */
+          for (d = 0; d < D; d++) b[d] += dr[d] * 0.5;
+          binary_reorient(w, f, b, P->division_r0, key, ts, 0);
+          reset_cell_motion(w, V, i);
+          V->tswitch[i] += random_span(w, &P->migration_tau, key, ts, 4, 1);
+          V->state[i] = CS_CRAWLING;
+          for (e = 0; e < 2; e++) V->sigma[(size_t)i * 2 + e] = P->sigma[e];
+        }
+      } else if (V->tswitch[i] == ts) {
+        V->state[i] = CS_DIVIDING;
+        V->tswitch[i] += random_span(w, &P->division_tau, key, ts, 5, 0);
+        reset_cell_motion(w, V, i);
+      }
+    }
+  }
+  return 0;
+}
+
 int or_divisions(or_ws* w, cil_species_view* v, int nspec, int ts) {
   if (w->p.division == CIL_DIV_THETA_CYCLE) return divisions_theta(w, v, nspec, ts);
   if (w->p.division == CIL_DIV_LINEAR_GROWTH) return divisions_linear(w, v, nspec, ts);
+  if (w->p.division == CIL_DIV_BINARY) return divisions_binary(w, v, nspec, ts);
+  return 0;
+}
+
+/* updateDiameters, CIL.cu:970-990: in a dividing species with DIVISION.SWELLING, every disc of
+ * a Dividing cell smaller than the species' largest default diameter grows by
+ * (max - sigma) / (tswitch - ts), so that it reaches the maximum at the switch time. */
+int or_update_diameters(or_ws* w, cil_species_view* v, int nspec, int ts) {
+  int s, i, e, nE = w->nE;
+  if (w->p.division == CIL_DIV_NONE) return 0;
+  for (s = 0; s < nspec; s++) {
+    const cil_species* P = &w->p.species[s];
+    cil_species_view* V = &v[s];
+    double smax = P->sigma[0];
+    if (P->no_division || !P->division_swelling) continue;
+    for (e = 1; e < nE; e++) smax = P->sigma[e] > smax ? P->sigma[e] : smax;
+    for (i = 0; i < V->ncells; i++)
+      if (V->state[i] == CS_DIVIDING)
+        for (e = 0; e < nE; e++) {
+          double* sg = &V->sigma[(size_t)i * nE + e];
+          if (*sg < smax) *sg += (smax - *sg) / (V->tswitch[i] - ts);
+        }
+  }
   return 0;
 }
 
@@ -1379,13 +1536,42 @@ int or_check_positions(or_ws* w, cil_species_view* v, int nspec, int* removed) {
 /* ===================================================================================== */
 /* Euler, CIL.cu:1013-1020                                                                */
 /* ===================================================================================== */
+/* updatePositions (CIL.cu:849-866): x += deltaT * zetaInv * F (zetaInv = 1/zeta, Input.cpp:93),
+ * wrapped; velocities untouched. */
+static void heun_half_update(or_ws* w, cil_species_view* v, int nspec) {
+  int s, c, e, D = w->box.dim, nE = w->nE;
+  const double hdt = 0.5 * w->p.dt; /* paramsMD.hdt, CIL.cuh:208 */
+  for (s = 0; s < nspec; s++) {
+    const cil_species* P = &w->p.species[s];
+    for (c = 0; c < v[s].ncells; c++)
+      for (e = 0; e < nE; e++) {
+        size_t pidx = (size_t)c * nE + e;
+        double temp = hdt * (1.0 / P->zeta[e]);
+        box_update_pos(&w->box, &v[s].pos[pidx * D], &v[s].frc[pidx * D], temp);
+      }
+  }
+}
+
 int or_step(or_ws* w, cil_species_view* v, int nspec, int nsteps, int* ts) {
   int k, rc;
   for (k = 0; k < nsteps; k++) {
-    update_positions(w, v, nspec, *ts);
+    if (w->p.integrator == CIL_INT_HEUN) {
+      /* StochasticHeun, CIL.cu:1028-1049.  The predictor's full step into pos0 is never read
+       * (pos0 is overwritten next step); the half step with the old forces, the forces at the
+       * half-step positions and a second half step with them give the new positions. */
+      heun_half_update(w, v, nspec);
+      w->phase = 1;
+      rc = or_forces(w, v, nspec, *ts);
+      w->phase = 0;
+      if (rc) return rc;
+      heun_half_update(w, v, nspec);
+    } else {
+      update_positions(w, v, nspec, *ts);
+    }
     /* calculComs runs here in the reference; the link list recomputes them when it is built. */
     ++(*ts);
     if ((rc = or_divisions(w, v, nspec, *ts))) return rc;
+    if ((rc = or_update_diameters(w, v, nspec, *ts))) return rc;
     if ((rc = or_forces(w, v, nspec, *ts))) return rc;
   }
   return 0;

@@ -67,7 +67,8 @@ int or_step(or_ws* w, cil_species_view* v, int nspec, int nsteps, int* ts);
 /* Pieces, exposed for unit tests. */
 int or_update(or_ws* w, cil_species_view* v, int nspec, int ts);         /* CIL.cu:902-939 */
 int or_coms(or_ws* w, cil_species_view* v, int nspec, double* com);       /* Colony.cpp:409-432 */
-int or_divisions(or_ws* w, cil_species_view* v, int nspec, int ts);        /* CIL.cu:469-619 */
+int or_divisions(or_ws* w, cil_species_view* v, int nspec, int ts);        /* CIL.cu:369-455, 469-619 */
+int or_update_diameters(or_ws* w, cil_species_view* v, int nspec, int ts); /* CIL.cu:970-990 */
 int or_check_positions(or_ws* w, cil_species_view* v, int nspec, int* removed); /* CIL.cu:945-966 */
 
 /* Interacting element pairs (UID_i < UID_j), sorted ascending.  mode 0: via the COM link

@@ -213,3 +213,65 @@ def test_list_mode_rejects_fp64(gpu):
     with pytest.raises(abi.CilError):
         md.set_neighbour_list(True)
     md.close()
+
+
+def _theta_growing(n_cells=120, L=24.0, seed=7):
+    """Single-disc Hertz colony in the theta cell cycle with long cycles (1.5 time units =
+    1500 steps) and phases in [0, 0.55): about 700 steps of diameter growth with no division
+    and no motility, so nothing but the growth can make a list stale."""
+    p, box, col = configs.theta_2d_hz(n_cells=n_cells, L=L, seed=seed, dt=1e-3, capacity=1024,
+                                      precision=abi.PREC_FP32)
+    sp = p.species[0]
+    sp.motility[0] = 0.0
+    sp.division_tau.mu = 1.5
+    sp.division_tau.sigma = 0.1
+    rng = np.random.default_rng(seed)
+    s = col.species[0]
+    s["theta"][:n_cells] = rng.uniform(0.0, 0.55, n_cells)
+    s["ang_vel"][:n_cells] = 1.0 / 1500.0
+    return p, box, col
+
+
+def test_list_with_theta_cycle_growth_matches_stencil(gpu):
+    """Neighbour lists under theta-cycle diameter growth: the cut-off of every pair grows with
+    the diameters, so the growth since the build is charged against the skin with the
+    displacement.  With a skin of 0.05 sigma the cut-offs outgrow it within ~100 steps while
+    no particle moves; a list that only checked displacement would miss the new contacts and
+    the discs would overlap without force.  Over 1000 steps (divisions from step ~700 on) the
+    list run must track the FP32 stencil run: same cell counts, positions within 1e-4 sigma,
+    and the end-state forces must hold every contact (within 1e-3 of the force scale of the
+    FP64 oracle)."""
+    p, box, col = _theta_growing()
+    out = []
+    for use_list in (False, True):
+        md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+        md.upload(col)
+        if use_list:
+            md.set_neighbour_list(True, 0.05, 64)
+        md.forces(want=False)
+        ts = 0
+        for _ in range(10):
+            ts = md.step(100, ts)
+        c = col.copy()
+        md.download(c)
+        md.close()
+        out.append(c)
+    n0, n1 = out[0].ncells()[0], out[1].ncells()[0]
+    say("theta + list: cells", n0, n1)
+    assert n0 == n1 and n0 > 120
+    s0, s1 = out[0].species[0], out[1].species[0]
+    dth = float(np.max(np.abs(s0["theta"][:n0] - s1["theta"][:n1])))
+    assert dth <= 1e-6, dth
+    err = _max_disp_err(out[0], out[1], box)
+    dsig = float(np.max(np.abs(s0["sigma"][:n0] - s1["sigma"][:n1])))
+    say(f"list vs stencil: positions {err:.3e}, diameters {dsig:.3e}; "
+        f"diameters grew to {float(np.max(s1['sigma'][:n1])):.3f}")
+    assert err <= 1e-4 and dsig <= 1e-6
+    # end-state forces against the FP64 oracle: a contact missing from the list would be an error
+    # of the order of its own term; FP32 rounding of shallow Hertz overlaps between discs of
+    # diameter ~1.6 (h = sigma_bar - r rounded to ~1e-7) stays below 1e-3 of the force scale
+    q = abi.Params.from_buffer_copy(p)
+    q.division = abi.DIV_NONE
+    worst = _force_bar(q, box, out[1], out[1].flat("frc").reshape(-1, 2), tol=1e-3)
+    say(f"end-state list forces vs FP64 oracle (bar 1e-3): worst {worst:.3f}")
+    assert worst <= 1.0, worst

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from cil_cuda_amd import abi, configs
-from cil_cuda_amd.colony import HipMD
+from cil_cuda_amd.colony import Colony, HipMD
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -299,6 +299,38 @@ def test_theta_cycle_divisions_bit_exact(gpu):
         assert np.array_equal(a[f][:n], b[f][:n]), f
     for f in ("theta", "rho", "ang_vel", "sigma_t"):
         assert np.allclose(a[f][:n], b[f][:n], rtol=1e-11, atol=1e-12), f
+    assert _max_disp_err(ref, got, box) <= 1e-9
+
+
+def test_theta_cycle_non_dividing_species(gpu):
+    """Two species in the theta cell cycle, the second one non-dividing (CellParams::DIV false,
+    divisionFunc NULL): only species 0 runs the cycle and divides (CIL.cu:496); species 1 keeps
+    its count and its cycle variables, its pressure timer still counts (CIL.cu:492-493).
+    Events are bit-exact against the oracle."""
+    say("theta divisions, mixed dividing / non-dividing species")
+    p, box, c1 = configs.theta_2d_hz(n_cells=24, L=30.0, seed=5)
+    p.nspecies = 2
+    abi_sp = p.species
+    ctypes_copy = abi.Species.from_buffer_copy(abi_sp[0])
+    abi_sp[1] = ctypes_copy
+    abi_sp[1].no_division = 1
+    configs.set_law(p, abi.LAW_HZ, 1.0)
+    col = Colony(2, 1, [256, 256], ncells=[12, 12])
+    for s_ in range(2):
+        for f in ("pos", "sigma", "sigma_t", "state", "rho", "theta", "ang_vel"):
+            col.species[s_][f][:12] = c1.species[0][f][12 * s_:12 * (s_ + 1)]
+    th1 = col.species[1]["theta"][:12].copy()
+    ref, got = _trajectory(p, box, col, 200)
+    assert ref.ncells() == got.ncells()
+    assert ref.ncells()[0] > 12 and ref.ncells()[1] == 12, ref.ncells()
+    assert np.array_equal(got.species[1]["theta"][:12], th1)
+    for s_ in range(2):
+        a, b = ref.species[s_], got.species[s_]
+        n = a["ncells"]
+        for f in ("state", "t0", "tswitch", "tpressure"):
+            assert np.array_equal(a[f][:n], b[f][:n]), (s_, f)
+        for f in ("theta", "rho", "ang_vel", "sigma_t"):
+            assert np.allclose(a[f][:n], b[f][:n], rtol=1e-11, atol=1e-12), (s_, f)
     assert _max_disp_err(ref, got, box) <= 1e-9
 
 
