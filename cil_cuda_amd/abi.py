@@ -24,7 +24,7 @@ SPRING_FENE, SPRING_HARMONIC = 0, 1
 # cil_noise
 NOISE_NONE, NOISE_CELL_WIENER, NOISE_FDT = 0, 1, 2
 # cil_rng
-RNG_COUNTER, RNG_EXTERNAL = 0, 1
+RNG_COUNTER, RNG_EXTERNAL, RNG_DSFMT = 0, 1, 2
 # cil_division
 DIV_NONE, DIV_THETA_CYCLE, DIV_LINEAR_GROWTH, DIV_BINARY = 0, 1, 2, 3
 # cil_integrator
@@ -89,7 +89,14 @@ class Params(C.Structure):
         ("drain", C.c_int), ("precision", C.c_int), ("pressure", C.c_int),
         ("max_disp", C.c_double),
         ("integrator", C.c_int),
+        ("rng_offset", C.c_uint64),
     ]
+
+
+class Run(C.Structure):
+    _fields_ = [("nspecies", C.c_int), ("gts", C.c_int), ("frames", C.c_int), ("ncells", C.c_int * MAX_SPECIES),
+                ("species", (C.c_char * 32) * MAX_SPECIES), ("elements", ((C.c_char * 32) * MAX_ELEMS) * MAX_SPECIES),
+                ("kbt", C.c_double), ("seed", C.c_long)]
 
 
 class Box(C.Structure):
@@ -186,6 +193,11 @@ def lib():
     L.cil_hip_last_error.argtypes = [vp]
     L.cil_hip_last_error.restype = C.c_char_p
     L.cil_hip_version.restype = C.c_char_p
+    L.cil_hip_config_read.argtypes = [C.c_char_p, C.POINTER(Params), C.POINTER(Box), C.POINTER(Run), C.c_char_p,
+                                      C.c_size_t]
+    L.cil_hip_config_init.argtypes = [C.c_char_p, C.POINTER(Params), C.POINTER(SpeciesView), C.c_int, C.c_char_p,
+                                      C.c_size_t]
+    L.cil_hip_dsfmt_fill.argtypes = [C.c_uint32, C.c_uint64, C.c_int, PD, C.c_size_t]
     L.cil_hip_sizeof.argtypes = [C.c_int]
     L.cil_hip_sizeof.restype = C.c_size_t
     _lib = L
@@ -194,7 +206,7 @@ def lib():
 
 def check_layout(L=None):
     L = L or lib()
-    want = {0: Params, 1: Species, 2: SpeciesView, 3: Box, 4: SurrogateStats}
+    want = {0: Params, 1: Species, 2: SpeciesView, 3: Box, 4: SurrogateStats, 5: Run}
     for k, cls in want.items():
         got = L.cil_hip_sizeof(k)
         if got != C.sizeof(cls):
@@ -208,6 +220,7 @@ EXPORTS = (
     "cil_hip_num_cells", "cil_hip_timings", "cil_hip_bin_info", "cil_hip_set_tuning", "cil_hip_set_timing", "cil_hip_set_kernel", "cil_hip_autotune",
     "cil_hip_set_neighbour_list",
     "cil_hip_sizeof", "cil_hip_last_error", "cil_hip_version",
+    "cil_hip_config_read", "cil_hip_config_init", "cil_hip_dsfmt_fill",
     "cil_hip_batch_create", "cil_hip_batch_destroy", "cil_hip_batch_size", "cil_hip_batch_step",
     "cil_hip_batch_autotune", "cil_hip_batch_last_error",
     "cil_hip_mlp_nparams", "cil_hip_mlp_set_params", "cil_hip_mlp_get_params", "cil_hip_mlp_init",

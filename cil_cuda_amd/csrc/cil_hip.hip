@@ -21,6 +21,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <ctime>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1932,7 +1934,9 @@ __global__ void k_pairs_list(const StepParams* __restrict__ Pp, Dev S, int reach
 }
 
 #include "cil_division.inc"
+#include "cil_dsfmt.inc"
 #include "cil_surrogate.inc"
 #include "cil_context.inc"
 #include "cil_batch.inc"
 #include "cil_surrogate_host.inc"
+#include "cil_config.inc"

@@ -15,6 +15,7 @@
 //   fpair/ppair : [D][P] / [P] pair forces and pressure partials, in sorted order
 #pragma once
 #include "device.hpp"
+#include "dsfmt.hpp"
 
 namespace cil {
 
@@ -77,6 +78,10 @@ struct Dev {
   // next index with phase 0).  `phase` is set per launch.
   int heun;
   int phase;
+  // CIL_RNG_DSFMT: the reference's sequential stream (one state, drawn by one device thread)
+  // and each cell's draws of the current step's division event, [C][8]
+  Dsfmt* dsfmt;
+  double* ddraw;
 };
 
 // ---------------------------------------------------------------------------------------

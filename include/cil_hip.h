@@ -66,7 +66,12 @@ typedef enum {
 
 typedef enum {
   CIL_RNG_COUNTER = 0,  /* Philox4x32-10 keyed by (seed, step, id, stream) */
-  CIL_RNG_EXTERNAL = 1  /* caller supplies the N(0,1) deviates (e.g. the reference's dSFMT stream) */
+  CIL_RNG_EXTERNAL = 1, /* caller supplies the N(0,1) deviates (e.g. the reference's dSFMT stream) */
+  CIL_RNG_DSFMT = 2     /* the reference's own sequential stream (Random::dsfmt, Random.hpp:8-52):
+                           dSFMT-19937 seeded with `seed`, advanced by `rng_offset` outputs, drawn
+                           on the device in the reference's order (Wiener noise, zero-velocity and
+                           zero-bond directions, division geometry and timers).  A replay mode: the
+                           draws are sequential, one device thread */
 } cil_rng;
 
 typedef enum {
@@ -178,6 +183,8 @@ typedef struct {
   double max_disp;                  /* extension for relaxing random initial states: per-step displacement
                                        cap (0 = off, the reference behaviour) */
   int integrator;                   /* cil_integrator (MDParams::INTEGRATOR, Input.cpp:520-540) */
+  uint64_t rng_offset;              /* CIL_RNG_DSFMT: outputs of the stream consumed before the first
+                                       draw of this context (e.g. by initializeCellStatus) */
 } cil_params;
 
 typedef struct {
@@ -205,6 +212,18 @@ typedef struct {
   double* ang_vel;   /* ncells */
   double* rest_len;  /* ncells  (linear-growth / harmonic extension) */
 } cil_species_view;
+
+/* A reference run's settings beyond cil_params (cil_hip_config_read). */
+typedef struct {
+  int nspecies;
+  int gts;                               /* MD.GTS: steps per frame */
+  int frames;                            /* MD.FRAMES */
+  int ncells[CIL_MAX_SPECIES];           /* cells the INIT block places per species */
+  char species[CIL_MAX_SPECIES][32];     /* SPECIES.DEFINE names */
+  char elements[CIL_MAX_SPECIES][CIL_MAX_ELEMS][32]; /* SPECIES.<name>.NAMES */
+  double kbt;                            /* MD.KBT (read; only the DPD integrator uses it) */
+  long seed;                             /* RUNTIME.RNG.SEED as used (<= 0: time(NULL), Input.cpp:577-583) */
+} cil_run;
 
 typedef struct cil_ctx cil_ctx;
 
@@ -288,8 +307,33 @@ int cil_hip_set_timing(cil_ctx* ctx, int on);
  * cut-off + skin while it is on; cil_hip_autotune requires it off. */
 int cil_hip_set_neighbour_list(cil_ctx* ctx, int on, double skin, int max_neighbours);
 
+/* ---------------------------------------------------------------------------------------
+ * Reference run configurations (project addition: the reference's own JSON input files).
+ *
+ * cil_hip_config_read parses a configuration in the reference's format (readMD, readSpecs,
+ * readInteractions, readRunTime: Input.cpp:6-590; initializeTimeStep: CIL.cuh:167-250) into
+ * cil_params / cil_box and the run settings.  The parameters come out FP64, with the reference's
+ * noise (Euler: none; STOCHASTIC_EULER: per-cell Wiener; STOCHASTIC_HEUN: Wiener and active
+ * noise), its division model (Euler: theta cell cycle; Heun: two-disc cycle) and its random
+ * stream (CIL_RNG_DSFMT seeded by RUNTIME.RNG.SEED).  Supported: UNITS all 1, DT MANUAL (or
+ * AUTOSCALE with LJ / SC), the EULER / STOCHASTIC_EULER / STOCHASTIC_HEUN integrators, the LJ /
+ * SC / HZ / EA laws, SLAB walls, one- or two-element species with FENE bonds, division and no
+ * apoptosis (EXTFORCE adds nothing, as in the reference, CIL.cu:55-66), INIT JSON or RANDOM.  Anything else returns CIL_E_INVALID with
+ * the reason in err.
+ * cil_hip_config_init builds the initial colony into the views (capacity >= run.ncells):
+ * readInit, initializeCellStatus (CIL.cuh:253-330) on the reference's stream, readInit again
+ * (CIL.cuh:372-376), and stores the stream outputs consumed in params->rng_offset, so that a
+ * context created from params continues the reference's stream where the initialisation left it.
+ * ------------------------------------------------------------------------------------- */
+int cil_hip_config_read(const char* json, cil_params* params, cil_box* box, cil_run* run, char* err, size_t errcap);
+int cil_hip_config_init(const char* json, cil_params* params, cil_species_view* views, int nspec, char* err,
+                        size_t errcap);
+/* n draws of the reference's stream (dSFMT-19937 from seed, after `skip` outputs): kind 0
+ * Random::uniform0x(1), 1 Random::normal, 2 dsfmt_genrand_open_open (project addition). */
+int cil_hip_dsfmt_fill(uint32_t seed, uint64_t skip, int kind, double* out, size_t n);
+
 /* sizeof the ABI structs (0: cil_params, 1: cil_species, 2: cil_species_view, 3: cil_box,
- * 4: cil_surrogate_stats),
+ * 4: cil_surrogate_stats, 5: cil_run),
  * so an FFI binding can verify its layout (project addition). */
 size_t cil_hip_sizeof(int which);
 

@@ -283,6 +283,7 @@ struct or_ws {
   double* com;
   int com_cap;
   int phase; /* 1 while the Heun corrector's forces are evaluated (stream offset ST_PHASE) */
+  int sample; /* threaded gather: pair forces only for cells whose UID is a multiple of sample (0/1: all) */
   char err[256];
 };
 
@@ -308,6 +309,12 @@ or_ws* or_create(const cil_params* p, const cil_box* box) {
   }
   w->nE = p->species[0].nelem;
   w->rng_mode = p->rng == CIL_RNG_EXTERNAL ? OR_RNG_EXTERNAL : OR_RNG_COUNTER;
+  if (p->rng == CIL_RNG_DSFMT) { /* the reference's stream from `seed`, past rng_offset outputs */
+    uint64_t k;
+    w->rng_mode = OR_RNG_DSFMT;
+    or_dsfmt_init(&w->g, (uint32_t)p->seed);
+    for (k = 0; k < p->rng_offset; k++) (void)ds_next_bits(&w->g);
+  }
   w->traversal = 0;
   w->nthreads = 1;
   return w;
@@ -334,6 +341,7 @@ void or_set_rng(or_ws* w, int mode, uint32_t seed) {
 }
 double or_rng_draw(or_ws* w, int kind) { return kind == 1 ? or_dsfmt_normal(&w->g) : or_dsfmt_close_open(&w->g); }
 void or_set_traversal(or_ws* w, int mode) { w->traversal = mode; }
+void or_set_sample(or_ws* w, int every) { w->sample = every; }
 int or_set_threads(or_ws* w, int n) {
   w->nthreads = n < 1 ? 1 : n;
   return w->nthreads;
@@ -548,7 +556,11 @@ static void elem_pair(or_ws* w, int ti, int tj, int spi, int spj, const double* 
     double A, temp;
     if (!(r <= (si + sj) / 2)) return;
     if (D == 2 && rij[0] == 0 && rij[1] == 0) { /* MD.cu:321-331 */
-      double alpha = rng_uniform0x(w, OR_TWOPI, pid_i, (uint32_t)ts, ST_PAIR + ST_PHASE * w->phase, pid_j, 0);
+      /* counter draw in every RNG mode: the reference's host body that would draw it from the
+       * sequential stream is absent (its device body uses curand, MD.cu:321-331) */
+      double u2[2], alpha;
+      or_philox_uniform2(w->p.seed, pid_i, (uint32_t)ts, ST_PAIR + ST_PHASE * w->phase, pid_j, u2);
+      alpha = u2[0] * OR_TWOPI;
       rij[0] = sin(alpha);
       rij[1] = cos(alpha);
     }
@@ -988,7 +1000,7 @@ static int draw_noise(or_ws* w, cil_species_view* v, int nspec, int ts) {
  * pragmas but writes both cells of a pair; this gathers instead): every cell visits the full
  * 3^D box shell around its own box and accumulates only its own force and pressure.  Pair
  * terms are evaluated in canonical orientation (lower UID first).  Counter RNG only. */
-static void inter_forces_gather(or_ws* w, cil_species_view* v, int nspec, const uidmap* um, int ts) {
+static void inter_forces_gather(or_ws* w, cil_species_view* v, int nspec, const uidmap* um, int ts, double* scale) {
   int D = w->box.dim, nE = w->nE, ncellsTot = um->cumul[nspec], nboxes = 0;
   int disabled = build_linklist(w, v, nspec, um, &nboxes);
   int ii;
@@ -996,6 +1008,7 @@ static void inter_forces_gather(or_ws* w, cil_species_view* v, int nspec, const 
   for (ii = 0; ii < ncellsTot; ii++) {
     int si, ci, icell[3] = {0, 0, 0}, ox, oy, oz, d, ib = 0;
     double dumf[3 * CIL_MAX_ELEMS], dump = 0.0;
+    if (w->sample > 1 && ii % w->sample != 0) continue;
     uid2cell(um, ii, &si, &ci);
     if (!disabled) {
       const double* cm = &w->com[(size_t)ii * D];
@@ -1027,6 +1040,14 @@ static void inter_forces_gather(or_ws* w, cil_species_view* v, int nspec, const 
               for (ej = 0; ej < nE; ej++) {
                 size_t pi = (size_t)ci * nE + ei, pj = (size_t)cj * nE + ej;
                 uint32_t ki = (uint32_t)((cell_key(si, ci) << 1) | ei), kj = (uint32_t)((cell_key(sj, cj) << 1) | ej);
+                if (scale) { /* the force scale of or_force_scale, gathered */
+                  double rij[3], f0, r2 = box_distance(&w->box, rij, &v[si].pos[pi * D], &v[sj].pos[pj * D]);
+                  int ti = si * nE + ei, tj = sj * nE + ej;
+                  if (w->p.law[ti][tj] != CIL_LAW_NONE &&
+                      or_pair_f0(&w->p, D, ti, tj, v[si].sigma[pi], v[sj].sigma[pj], r2, &f0))
+                    scale[(size_t)ii * nE + ei] += fabs(f0) * sqrt(r2) + 1e-300;
+                  continue;
+                }
                 for (d = 0; d < 3 * CIL_MAX_ELEMS; d++) dumf[d] = 0.0;
                 if (ii < jj)
                   elem_pair(w, si * nE + ei, sj * nE + ej, si, sj, &v[si].pos[pi * D], &v[sj].pos[pj * D],
@@ -1051,7 +1072,7 @@ static void inter_forces(or_ws* w, cil_species_view* v, int nspec, int ts) {
   for (s = 0; s < nspec; s++)
     for (c = 0; c < v[s].ncells; c++) v[s].pressure[c] = 0;
   if (w->nthreads > 1 && w->traversal == 0 && w->rng_mode == OR_RNG_COUNTER) {
-    inter_forces_gather(w, v, nspec, &um, ts);
+    inter_forces_gather(w, v, nspec, &um, ts, NULL);
     return;
   }
   traverse(w, v, nspec, &um, force_cb, &ts);
@@ -1640,6 +1661,16 @@ void or_linklist_info(or_ws* w, int* ns, double* width) {
 }
 
 /* Per-particle sum of |pair force contributions| (all-pairs), for relative tolerances. */
+int or_force_scale_gather(or_ws* w, cil_species_view* v, int nspec, double* scale) {
+  uidmap um;
+  w->err[0] = 0;
+  if (w->traversal != 0) return fail(w, CIL_E_INVALID, "force_scale_gather needs the link-list traversal");
+  make_uidmap(&um, v, nspec);
+  memset(scale, 0, sizeof(double) * (size_t)um.cumul[nspec] * w->nE);
+  inter_forces_gather(w, v, nspec, &um, 0, scale);
+  return 0;
+}
+
 int or_force_scale(or_ws* w, cil_species_view* v, int nspec, double* scale) {
   uidmap um;
   int D = w->box.dim, nE = w->nE, ii, jj, ntot;

@@ -58,6 +58,9 @@ int or_set_external_noise(or_ws* w, const double* normals, size_t per_step, int 
 /* Pair-traversal back end for inter forces: 0 = COM link list (reference structure,
  * CIL.cu:177-231), 1 = all pairs (CIL.cu:232-263). */
 void or_set_traversal(or_ws* w, int mode);
+/* threaded gather (threads > 1, counter RNG): pair forces of the cells whose UID is a multiple of
+ * `every` only, the other cells keep their intra forces (a sample of a large colony) */
+void or_set_sample(or_ws* w, int every);
 int or_set_threads(or_ws* w, int nthreads); /* OpenMP gather variant when > 1 */
 
 /* calculForcesPressure, CIL.cu:999-1007 (draws noise when enabled). */
@@ -76,6 +79,9 @@ int or_check_positions(or_ws* w, cil_species_view* v, int nspec, int* removed); 
 long or_pairs(or_ws* w, cil_species_view* v, int nspec, int mode, int32_t* ij, long cap);
 
 /* Per-particle force scale sum_j |f_ij| (norm of each pair term), for relative tolerances. */
+/* or_force_scale over the link list with the threaded gather (the sample of or_set_sample, the
+ * other entries stay 0); traversal 0 only */
+int or_force_scale_gather(or_ws* w, cil_species_view* v, int nspec, double* scale);
 int or_force_scale(or_ws* w, cil_species_view* v, int nspec, double* scale);
 
 /* Scalar pair law f0 for element types ti,tj with diameters si,sj at squared distance r2

@@ -37,6 +37,7 @@ def lib(fast: bool = False):
     L.or_set_external_noise.argtypes = [vp, abi.PD, C.c_size_t, C.c_int]
     L.or_set_traversal.argtypes = [vp, C.c_int]
     L.or_set_threads.argtypes = [vp, C.c_int]
+    L.or_set_sample.argtypes = [vp, C.c_int]
     L.or_forces.argtypes = [vp, PV, C.c_int, C.c_int]
     L.or_step.argtypes = [vp, PV, C.c_int, C.c_int, abi.PI]
     L.or_update.argtypes = [vp, PV, C.c_int, C.c_int]
@@ -46,6 +47,7 @@ def lib(fast: bool = False):
     L.or_pairs.argtypes = [vp, PV, C.c_int, C.c_int, C.POINTER(C.c_int32), C.c_long]
     L.or_pairs.restype = C.c_long
     L.or_force_scale.argtypes = [vp, PV, C.c_int, abi.PD]
+    L.or_force_scale_gather.argtypes = [vp, PV, C.c_int, abi.PD]
     L.or_pair_f0.argtypes = [C.POINTER(abi.Params), C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                              abi.PD]
     L.or_linklist_info.argtypes = [vp, abi.PI, abi.PD]
@@ -96,6 +98,10 @@ class Oracle:
         if rc != 0:
             raise OracleError(f"{what}: {rc} {self.L.or_last_error(self.w).decode()}")
 
+    def set_sample(self, every):
+        """Threaded gather only: pair forces of every `every`-th cell (by UID) only."""
+        self.L.or_set_sample(self.w, int(every))
+
     def set_rng(self, mode, seed=0):
         self.L.or_set_rng(self.w, mode, seed)
 
@@ -133,6 +139,14 @@ class Oracle:
         n = col.total_cells() * col.nelem
         out = np.zeros(max(n, 1))
         self.L.or_force_scale(self.w, v, col.nspec, out.ctypes.data_as(abi.PD))
+        return out[:n]
+
+    def force_scale_gather(self, col):
+        """force_scale over the link list, threaded, for the sample of set_sample (others 0)."""
+        v = col.views()
+        n = col.total_cells() * col.nelem
+        out = np.zeros(max(n, 1))
+        self._chk(self.L.or_force_scale_gather(self.w, v, col.nspec, out.ctypes.data_as(abi.PD)), "force_scale_gather")
         return out[:n]
 
     def check_positions(self, col):
