@@ -156,6 +156,7 @@ def lib():
     L.cil_hip_download.argtypes = [vp, C.POINTER(SpeciesView), C.c_int, PI]
     L.cil_hip_forces.argtypes = [vp, PD, PD]
     L.cil_hip_pairs.argtypes = [vp, C.POINTER(C.c_int32), C.c_size_t, C.POINTER(C.c_size_t)]
+    L.cil_hip_pairs_noalloc.argtypes = [vp, C.POINTER(C.c_int32), C.c_size_t, C.POINTER(C.c_size_t)]
     L.cil_hip_check_positions.argtypes = [vp, PI]
     L.cil_hip_set_external_noise.argtypes = [vp, PD, C.c_size_t, C.c_int]
     L.cil_hip_num_cells.argtypes = [vp, PI]
@@ -216,7 +217,7 @@ def check_layout(L=None):
 
 EXPORTS = (
     "cil_hip_create", "cil_hip_destroy", "cil_hip_upload", "cil_hip_step", "cil_hip_download",
-    "cil_hip_forces", "cil_hip_pairs", "cil_hip_check_positions", "cil_hip_set_external_noise",
+    "cil_hip_forces", "cil_hip_pairs", "cil_hip_pairs_noalloc", "cil_hip_check_positions", "cil_hip_set_external_noise",
     "cil_hip_num_cells", "cil_hip_timings", "cil_hip_bin_info", "cil_hip_set_tuning", "cil_hip_set_timing", "cil_hip_set_kernel", "cil_hip_autotune",
     "cil_hip_set_neighbour_list",
     "cil_hip_sizeof", "cil_hip_last_error", "cil_hip_version",

@@ -1939,4 +1939,5 @@ __global__ void k_pairs_list(const StepParams* __restrict__ Pp, Dev S, int reach
 #include "cil_context.inc"
 #include "cil_batch.inc"
 #include "cil_surrogate_host.inc"
+#include "cil_noalloc.inc"
 #include "cil_config.inc"

@@ -253,6 +253,14 @@ int cil_hip_forces(cil_ctx* ctx, double* frc, double* pressure);
  * in ascending (i, j) order.  Writes up to `cap` pairs as ij[2k], ij[2k+1]; *n = total. */
 int cil_hip_pairs(cil_ctx* ctx, int32_t* ij, size_t cap, size_t* n);
 
+/* cil_hip_pairs without device allocation (no counterpart in the reference): for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same result as cil_hip_pairs; the pairs and their counter are staged in a
+ * fixed device array of the library (one per GPU, held until the process ends) instead of a
+ * buffer allocated to `cap`.  cap <= 1048576 pairs, else CIL_E_INVALID.  Calls on one GPU share
+ * that array: calls that overlap on one GPU (contexts used from different threads) race. */
+int cil_hip_pairs_noalloc(cil_ctx* ctx, int32_t* ij, size_t cap, size_t* n);
+
 /* checkPositions (CIL.cu:945-966): out-of-box cells raise CIL_E_OUTSIDE unless drain,
  * in which case they are removed by swap-with-last with the reference's skip quirk. */
 int cil_hip_check_positions(cil_ctx* ctx, int* removed);
