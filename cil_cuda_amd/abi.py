@@ -16,7 +16,9 @@ MAX_WALLS = 4
 ABI_VERSION = 1
 
 # cil_status
-OK, E_INVALID, E_NO_DEVICE, E_CAPACITY, E_OUTSIDE, E_NONFINITE, E_HIP, E_BOND = 0, -1, -2, -3, -4, -5, -6, -7
+OK, E_INVALID, E_NO_DEVICE, E_CAPACITY, E_OUTSIDE, E_NONFINITE, E_HIP, E_BOND, E_IO = 0, -1, -2, -3, -4, -5, -6, -7, -8
+(GSD_UINT8, GSD_UINT16, GSD_UINT32, GSD_UINT64, GSD_INT8, GSD_INT16, GSD_INT32, GSD_INT64, GSD_FLOAT,
+ GSD_DOUBLE) = range(1, 11)
 # cil_law
 LAW_NONE, LAW_LJ, LAW_SC, LAW_HZ, LAW_EA = 0, 1, 2, 3, 4
 # cil_spring
@@ -199,6 +201,13 @@ def lib():
     L.cil_hip_config_init.argtypes = [C.c_char_p, C.POINTER(Params), C.POINTER(SpeciesView), C.c_int, C.c_char_p,
                                       C.c_size_t]
     L.cil_hip_dsfmt_fill.argtypes = [C.c_uint32, C.c_uint64, C.c_int, PD, C.c_size_t]
+    L.cil_hip_gsd_create.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint32, C.POINTER(vp)]
+    L.cil_hip_gsd_write_chunk.argtypes = [vp, C.c_char_p, C.c_int, C.c_uint64, C.c_uint32, vp]
+    L.cil_hip_gsd_end_frame.argtypes = [vp]
+    L.cil_hip_gsd_close.argtypes = [vp]
+    L.cil_hip_gsd_write_frame.argtypes = [vp, C.POINTER(Params), C.POINTER(Box), C.POINTER(Run), C.POINTER(SpeciesView),
+                                          C.c_int, C.c_int]
+    L.cil_hip_gsd_write_frame_ctx.argtypes = [vp, vp, C.POINTER(Run), C.c_int]
     L.cil_hip_sizeof.argtypes = [C.c_int]
     L.cil_hip_sizeof.restype = C.c_size_t
     _lib = L
@@ -221,7 +230,8 @@ EXPORTS = (
     "cil_hip_num_cells", "cil_hip_timings", "cil_hip_bin_info", "cil_hip_set_tuning", "cil_hip_set_timing", "cil_hip_set_kernel", "cil_hip_autotune",
     "cil_hip_set_neighbour_list",
     "cil_hip_sizeof", "cil_hip_last_error", "cil_hip_version",
-    "cil_hip_config_read", "cil_hip_config_init", "cil_hip_dsfmt_fill",
+    "cil_hip_config_read", "cil_hip_config_init", "cil_hip_dsfmt_fill", "cil_hip_gsd_create", "cil_hip_gsd_write_chunk",
+    "cil_hip_gsd_end_frame", "cil_hip_gsd_close", "cil_hip_gsd_write_frame", "cil_hip_gsd_write_frame_ctx",
     "cil_hip_batch_create", "cil_hip_batch_destroy", "cil_hip_batch_size", "cil_hip_batch_step",
     "cil_hip_batch_autotune", "cil_hip_batch_last_error",
     "cil_hip_mlp_nparams", "cil_hip_mlp_set_params", "cil_hip_mlp_get_params", "cil_hip_mlp_init",

@@ -1941,3 +1941,4 @@ __global__ void k_pairs_list(const StepParams* __restrict__ Pp, Dev S, int reach
 #include "cil_surrogate_host.inc"
 #include "cil_noalloc.inc"
 #include "cil_config.inc"
+#include "cil_output.inc"

@@ -40,7 +40,8 @@ typedef enum {
   CIL_E_OUTSIDE = -4,     /* particle outside the box and drain disabled (CIL.cu:955) */
   CIL_E_NONFINITE = -5,   /* NaN/Inf found in positions or forces */
   CIL_E_HIP = -6,         /* a HIP call failed; see cil_hip_last_error */
-  CIL_E_BOND = -7         /* FENE bond over-stretched (MD.cu:549 assert) */
+  CIL_E_BOND = -7,        /* FENE bond over-stretched (MD.cu:549 assert) */
+  CIL_E_IO = -8           /* a file could not be created or written */
 } cil_status;
 
 /* Pair-force law per element-type pair (reference: one global PairForceParams, MD.cuh:23-344;
@@ -339,6 +340,39 @@ int cil_hip_config_init(const char* json, cil_params* params, cil_species_view* 
 /* n draws of the reference's stream (dSFMT-19937 from seed, after `skip` outputs): kind 0
  * Random::uniform0x(1), 1 Random::normal, 2 dsfmt_genrand_open_open (project addition). */
 int cil_hip_dsfmt_fill(uint32_t seed, uint64_t skip, int kind, double* out, size_t n);
+
+/* ---------------------------------------------------------------------------------------
+ * Frames in the reference's GSD format (project addition: the reference writes them with the
+ * vendored gsd.c through outputGSDFrame, OutputWriter.cpp:505-657).
+ *
+ * cil_hip_gsd_create / _write_chunk / _end_frame / _close replace gsd_create + gsd_open
+ * (GSD_OPEN_APPEND), gsd_write_chunk, gsd_end_frame and gsd_close for writing (gsd.h:113-143):
+ * file format 1.0, the same layout and growth of the index and name list, so a file written
+ * chunk for chunk as the reference writes it is identical byte for byte.  Chunk types are
+ * gsd.h's gsd_type values (CIL_GSD_*).
+ *
+ * cil_hip_gsd_write_frame writes one frame of a host colony as outputGSDFrame does:
+ * configuration/step, /dimensions (3), /box (lx, ly, lz; lz in 2D = the largest species
+ * SIGMA_CORE, OutputWriter.cpp:715); particles/N; particles/types at ts == 0 ("species" or
+ * "species_element", 63 bytes, names from `run`, which may be NULL: C1, C2, ... and E0, E1);
+ * particles/Z, /diameter (sigma) and /typeid (species * nelem + element) when a species divides
+ * or ts == 0; particles/charge (rho) when a species divides; particles/position (centred on the
+ * box; in 2D z = -0.01 * element); particles/velocity; particles/mass (the cell pressure).
+ * cil_hip_gsd_write_frame_ctx does the same from a context's device colony. */
+typedef enum {
+  CIL_GSD_UINT8 = 1, CIL_GSD_UINT16 = 2, CIL_GSD_UINT32 = 3, CIL_GSD_UINT64 = 4,
+  CIL_GSD_INT8 = 5, CIL_GSD_INT16 = 6, CIL_GSD_INT32 = 7, CIL_GSD_INT64 = 8,
+  CIL_GSD_FLOAT = 9, CIL_GSD_DOUBLE = 10
+} cil_gsd_type;
+typedef struct cil_gsd cil_gsd;
+int cil_hip_gsd_create(const char* path, const char* application, const char* schema, uint32_t schema_version,
+                       cil_gsd** out);
+int cil_hip_gsd_write_chunk(cil_gsd* g, const char* name, int type, uint64_t N, uint32_t M, const void* data);
+int cil_hip_gsd_end_frame(cil_gsd* g);
+int cil_hip_gsd_close(cil_gsd* g);
+int cil_hip_gsd_write_frame(cil_gsd* g, const cil_params* params, const cil_box* box, const cil_run* run,
+                            const cil_species_view* views, int nspec, int ts);
+int cil_hip_gsd_write_frame_ctx(cil_ctx* ctx, cil_gsd* g, const cil_run* run, int ts);
 
 /* sizeof the ABI structs (0: cil_params, 1: cil_species, 2: cil_species_view, 3: cil_box,
  * 4: cil_surrogate_stats, 5: cil_run),
