@@ -98,7 +98,8 @@ class Params(C.Structure):
 class Run(C.Structure):
     _fields_ = [("nspecies", C.c_int), ("gts", C.c_int), ("frames", C.c_int), ("ncells", C.c_int * MAX_SPECIES),
                 ("species", (C.c_char * 32) * MAX_SPECIES), ("elements", ((C.c_char * 32) * MAX_ELEMS) * MAX_SPECIES),
-                ("kbt", C.c_double), ("seed", C.c_long)]
+                ("kbt", C.c_double), ("seed", C.c_long), ("sigma_max", (C.c_double * MAX_ELEMS) * MAX_SPECIES),
+                ("f_ext", (C.c_double * 3) * MAX_SPECIES)]
 
 
 class Box(C.Structure):
@@ -208,6 +209,11 @@ def lib():
     L.cil_hip_gsd_write_frame.argtypes = [vp, C.POINTER(Params), C.POINTER(Box), C.POINTER(Run), C.POINTER(SpeciesView),
                                           C.c_int, C.c_int]
     L.cil_hip_gsd_write_frame_ctx.argtypes = [vp, vp, C.POINTER(Run), C.c_int]
+    L.cil_hip_h5_create.argtypes = [C.c_char_p, C.POINTER(Params), C.POINTER(Box), C.POINTER(Run), C.POINTER(vp)]
+    L.cil_hip_h5_write_frame.argtypes = [vp, C.POINTER(Params), C.POINTER(SpeciesView), C.c_int, C.POINTER(Run), C.c_int,
+                                         C.c_int, C.c_double]
+    L.cil_hip_h5_write_frame_ctx.argtypes = [vp, vp, C.POINTER(Run), C.c_int, C.c_int, C.c_double]
+    L.cil_hip_h5_close.argtypes = [vp]
     L.cil_hip_sizeof.argtypes = [C.c_int]
     L.cil_hip_sizeof.restype = C.c_size_t
     _lib = L
@@ -232,6 +238,7 @@ EXPORTS = (
     "cil_hip_sizeof", "cil_hip_last_error", "cil_hip_version",
     "cil_hip_config_read", "cil_hip_config_init", "cil_hip_dsfmt_fill", "cil_hip_gsd_create", "cil_hip_gsd_write_chunk",
     "cil_hip_gsd_end_frame", "cil_hip_gsd_close", "cil_hip_gsd_write_frame", "cil_hip_gsd_write_frame_ctx",
+    "cil_hip_h5_create", "cil_hip_h5_write_frame", "cil_hip_h5_write_frame_ctx", "cil_hip_h5_close",
     "cil_hip_batch_create", "cil_hip_batch_destroy", "cil_hip_batch_size", "cil_hip_batch_step",
     "cil_hip_batch_autotune", "cil_hip_batch_last_error",
     "cil_hip_mlp_nparams", "cil_hip_mlp_set_params", "cil_hip_mlp_get_params", "cil_hip_mlp_init",

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <ctime>
 #include <mutex>
@@ -1942,3 +1943,4 @@ __global__ void k_pairs_list(const StepParams* __restrict__ Pp, Dev S, int reach
 #include "cil_noalloc.inc"
 #include "cil_config.inc"
 #include "cil_output.inc"
+#include "cil_h5.inc"

@@ -54,3 +54,43 @@ class GsdWriter:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class H5Writer:
+    """An HDF5 file in the reference's layout: /params from the run's parameters (or a bare file
+    when `params` is None, the reference's `.Last.h5`) and snapshot frames."""
+
+    def __init__(self, path, params=None, box=None, run=None):
+        self.L = abi.lib()
+        self.f = C.c_void_p()
+        rc = self.L.cil_hip_h5_create(str(path).encode(), C.byref(params) if params is not None else None,
+                                      C.byref(box) if box is not None else None,
+                                      C.byref(run) if run is not None else None, C.byref(self.f))
+        if rc != 0:
+            raise abi.CilError(rc, f"h5_create {path}")
+
+    def frame(self, params, col, frame_id, ts, run=None, cpu_time=0.0):
+        """/snapshots/t_<frame_id> (frame_id < 0: at the root) of a host colony."""
+        rc = self.L.cil_hip_h5_write_frame(self.f, C.byref(params), col.views(), col.nspec,
+                                           C.byref(run) if run is not None else None, frame_id, ts, cpu_time)
+        if rc != 0:
+            raise abi.CilError(rc, "h5 frame")
+
+    def frame_ctx(self, md, frame_id, ts, run=None, cpu_time=0.0):
+        rc = self.L.cil_hip_h5_write_frame_ctx(md.ctx, self.f, C.byref(run) if run is not None else None, frame_id, ts,
+                                               cpu_time)
+        if rc != 0:
+            raise abi.CilError(rc, "h5 frame")
+
+    def close(self):
+        if self.f:
+            rc = self.L.cil_hip_h5_close(self.f)
+            self.f = C.c_void_p()
+            if rc != 0:
+                raise abi.CilError(rc, "h5 close")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

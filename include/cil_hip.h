@@ -224,6 +224,9 @@ typedef struct {
   char elements[CIL_MAX_SPECIES][CIL_MAX_ELEMS][32]; /* SPECIES.<name>.NAMES */
   double kbt;                            /* MD.KBT (read; only the DPD integrator uses it) */
   long seed;                             /* RUNTIME.RNG.SEED as used (<= 0: time(NULL), Input.cpp:577-583) */
+  /* parameters the step does not use, kept for the run's HDF5 parameters */
+  double sigma_max[CIL_MAX_SPECIES][CIL_MAX_ELEMS]; /* PAIRFORCES.SC.SIGMA_MAX (0 for other laws) */
+  double f_ext[CIL_MAX_SPECIES][3];      /* EXTFORCE (the reference adds 0 for it, CIL.cu:55-66) */
 } cil_run;
 
 typedef struct cil_ctx cil_ctx;
@@ -373,6 +376,25 @@ int cil_hip_gsd_close(cil_gsd* g);
 int cil_hip_gsd_write_frame(cil_gsd* g, const cil_params* params, const cil_box* box, const cil_run* run,
                             const cil_species_view* views, int nspec, int ts);
 int cil_hip_gsd_write_frame_ctx(cil_ctx* ctx, cil_gsd* g, const cil_run* run, int ts);
+
+/* HDF5 snapshots in the reference's layout (project addition: initHDF5Output and
+ * outputHDF5Frame, OutputWriter.cpp:240-436, through the reference's SimIO wrapper).
+ * cil_hip_h5_create writes /params (dt, L, Tdump = run->gts, Force), /params/units (all 1) and
+ * /params/cell/<species> (flags, division and migration times, noise, element parameters,
+ * bond matrices; `run` may be NULL) and creates /snapshots; with params NULL it creates a bare
+ * file (the reference's `.Last.h5`).  cil_hip_h5_write_frame adds /snapshots/t_<id> (attribute
+ * ts), or for id < 0 writes at the root, R, Vel, SigmaCore, F, FD, FR, RHO, Theta, PR, State,
+ * T0 and CpuTime per species (a subgroup per species when there are several), reals as 32-bit
+ * floats.  FD and FR (the DPD forces) are zero.  The division / apoptosis records (/div_app) are
+ * not written.
+ * The HDF5 library is loaded at the first create ($CIL_HDF5_LIB, libhdf5.so, or
+ * /opt/conda/lib/libhdf5.so); without it the calls return CIL_E_IO. */
+typedef struct cil_h5 cil_h5;
+int cil_hip_h5_create(const char* path, const cil_params* params, const cil_box* box, const cil_run* run, cil_h5** out);
+int cil_hip_h5_write_frame(cil_h5* f, const cil_params* params, const cil_species_view* views, int nspec,
+                           const cil_run* run, int id, int ts, double cpu_time);
+int cil_hip_h5_write_frame_ctx(cil_ctx* ctx, cil_h5* f, const cil_run* run, int id, int ts, double cpu_time);
+int cil_hip_h5_close(cil_h5* f);
 
 /* sizeof the ABI structs (0: cil_params, 1: cil_species, 2: cil_species_view, 3: cil_box,
  * 4: cil_surrogate_stats, 5: cil_run),

@@ -5,7 +5,7 @@
   C2  2D channel 40 wide, one two-disc cell grown to 65,536 cells (Hertz, walls, drain, linear
       growth 1/tau to axis 2 sigma, +-5 % diameter jitter, dt 5e-5, kT 0.001): no capacity
       error, every drain removal a cell outside the box, and a bit-identical replay of the
-      first 10^5 steps.
+      first 10^5 steps; in FP64 the first 255 divisions equal the oracle's.
   C3  3D bulk of 1,048,576 discs (mixed LJ 2.5 sigma / soft core, phi 0.6): FP32 neighbour-list
       forces of the relaxed state against the FP64 oracle for 16,384 sampled particles, FP32 bar.
   C4  256 colonies of log-uniform 1k-64k two-disc cells (seed 1000 + c, motility U[0, 0.5],
@@ -67,9 +67,12 @@ def test_c1_full_size_matches_oracle(gpu):
     assert err <= 1e-9
 
 
-def _c2(length=8192.0):
-    return configs.channel_2d_hz(width=40.0, length=length, seed=1, kT=0.001, dt=5e-5, tau=1.0, capacity=70000,
-                                 precision=abi.PREC_FP32)
+C2_CAP = 131072  # divisions come in near-synchronous generations: room for the one that crosses 65,536
+
+
+def _c2(length=8192.0, precision=abi.PREC_FP32, capacity=C2_CAP):
+    return configs.channel_2d_hz(width=40.0, length=length, seed=1, kT=0.001, dt=5e-5, tau=1.0, capacity=capacity,
+                                 precision=precision)
 
 
 def test_c2_channel_grows_to_65536_cells(gpu):
@@ -80,12 +83,13 @@ def test_c2_channel_grows_to_65536_cells(gpu):
     a few tens of sigma long and the drain removes them.  Asserted: the drain removes only cells
     with a disc outside the box, and leaves none behind beyond the reference's swap-skip quirk."""
     p, box, col = _c2()
-    md = HipMD(p, box, [70000])
+    md = HipMD(p, box, [C2_CAP])
     md.upload(col)
     md.forces(want=False)
-    ts, chunk, removed_total = 0, 20000, 0
+    ts, removed_total, n = 0, 0, 1
     t0 = time.perf_counter()
     while True:
+        chunk = 20000 if n < 16384 else 2000  # stop within a few hundred cells of 65,536
         ts = md.step(chunk, ts)
         c = col.copy()
         md.download(c)
@@ -107,7 +111,7 @@ def test_c2_channel_grows_to_65536_cells(gpu):
         # every removed cell was outside, and every outside cell is removed or left by the quirk
         assert left == outside - removed and (outside == 0 or removed >= 1), (ts, outside, removed, left)
         removed_total += removed
-        if ts % 100000 == 0 or n >= 65536:
+        if ts % 100000 == 0 or n >= 65536 or (n >= 16384 and ts % 10000 == 0):
             say(f"C2: step {ts}: {n} cells, removed {removed_total} ({out_x} through the walls this check), "
                 f"{time.perf_counter() - t0:.1f} s")
         if n >= 65536:
@@ -117,7 +121,7 @@ def test_c2_channel_grows_to_65536_cells(gpu):
     # bit-identical replay of the first 10^5 steps
     outs = []
     for _ in range(2):
-        md = HipMD(p, box, [70000])
+        md = HipMD(p, box, [C2_CAP])
         md.upload(col)
         md.forces(want=False)
         md.step(100000, 0)
@@ -129,6 +133,36 @@ def test_c2_channel_grows_to_65536_cells(gpu):
     assert outs[0].ncells() == outs[1].ncells()
     for f in ("pos", "sigma", "rest_len", "t0"):
         assert np.array_equal(outs[0].flat(f), outs[1].flat(f)), f
+
+
+def test_c2_first_255_divisions_match_oracle(gpu):
+    """The channel from its one cell to 256 cells (255 divisions, 240,000 steps) in FP64 on the GPU
+    and in the oracle: the same division events (cell count, birth steps, jittered daughter
+    diameters and rest lengths bit for bit) and positions that agree to within 1e-3 sigma (the
+    two sum pair forces in different orders)."""
+    p, box, col = _c2(precision=abi.PREC_FP64, capacity=4096)
+    md = HipMD(p, box, [4096])
+    md.upload(col)
+    md.forces(want=False)
+    assert md.step(240000, 0) == 240000
+    got = md.download(col.copy())
+    md.close()
+    say(f"C2 FP64: GPU at step 240000: {got.ncells()} cells")
+    ref = col.copy()
+    orc = O.Oracle(p, box, traversal=0, threads=max(2, _threads()))
+    orc.forces(ref, 0)
+    t0, ts = time.perf_counter(), 0
+    while ts < 240000:
+        ts = orc.step(ref, 20000, ts)
+        say(f"C2 FP64: oracle step {ts}: {ref.ncells()} cells ({time.perf_counter() - t0:.1f} s)")
+    orc.close()
+    assert got.ncells() == ref.ncells() == [256]
+    a, b = got.species[0], ref.species[0]
+    for f in ("t0", "sigma", "rest_len", "state"):
+        assert np.array_equal(a[f][:256], b[f][:256]), f
+    err = _max_disp_err(ref, got, box)
+    say(f"C2 FP64: max position difference after 255 divisions: {err:.3e}")
+    assert err <= 1e-3
 
 
 def test_c3_1m_list_forces_match_fp64(gpu):
