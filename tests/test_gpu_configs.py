@@ -137,9 +137,9 @@ def test_c2_channel_grows_to_65536_cells(gpu):
 
 def test_c2_first_255_divisions_match_oracle(gpu):
     """The channel from its one cell to 256 cells (255 divisions, 240,000 steps) in FP64 on the GPU
-    and in the oracle: the same division events (cell count, birth steps, jittered daughter
-    diameters and rest lengths bit for bit) and positions that agree to within 1e-3 sigma (the
-    two sum pair forces in different orders)."""
+    and in the oracle: the same division events (cell count, birth steps and rest lengths bit for
+    bit, jittered daughter diameters to 1e-14) and positions that agree to within 1e-3 sigma
+    (the two sum pair forces in different orders)."""
     p, box, col = _c2(precision=abi.PREC_FP64, capacity=4096)
     md = HipMD(p, box, [4096])
     md.upload(col)
@@ -158,8 +158,13 @@ def test_c2_first_255_divisions_match_oracle(gpu):
     orc.close()
     assert got.ncells() == ref.ncells() == [256]
     a, b = got.species[0], ref.species[0]
-    for f in ("t0", "sigma", "rest_len", "state"):
+    for f in ("t0", "rest_len", "state"):
         assert np.array_equal(a[f][:256], b[f][:256]), f
+    # the jittered diameters come from the same Philox draws; the Gaussian transform's log / cos
+    # may differ in the last bit between the device and the host library
+    dsig = float(np.max(np.abs(a["sigma"][:256] / b["sigma"][:256] - 1.0)))
+    say(f"C2 FP64: daughter diameters: max relative difference {dsig:.2e}")
+    assert dsig <= 1e-14
     err = _max_disp_err(ref, got, box)
     say(f"C2 FP64: max position difference after 255 divisions: {err:.3e}")
     assert err <= 1e-3
