@@ -275,3 +275,42 @@ def test_list_with_theta_cycle_growth_matches_stencil(gpu):
     worst = _force_bar(q, box, out[1], out[1].flat("frc").reshape(-1, 2), tol=1e-3)
     say(f"end-state list forces vs FP64 oracle (bar 1e-3): worst {worst:.3f}")
     assert worst <= 1.0, worst
+
+
+def test_two_list_contexts_step_concurrently(gpu):
+    """Two list-mode contexts (each with its own stream and persistent sort kernel) stepped from
+    two host threads at once give the states they give when stepped one after the other."""
+    import threading
+
+    setups = [_relaxed(32768, s) for s in (11, 12)]
+
+    def make(k):
+        p, box, col = setups[k]
+        md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+        md.upload(col)
+        md.set_neighbour_list(True, 0.05, 256)
+        md.forces(want=False)
+        return md
+
+    def run(md, out, k):
+        ts = 0
+        for _ in range(10):
+            ts = md.step(50, ts)
+        out[k] = md.download(setups[k][2].copy()).flat("pos")
+
+    seq, par = {}, {}
+    for k in (0, 1):
+        md = make(k)
+        run(md, seq, k)
+        md.close()
+    mds = [make(0), make(1)]
+    th = [threading.Thread(target=run, args=(mds[k], par, k)) for k in (0, 1)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for md in mds:
+        md.close()
+    assert set(par) == {0, 1}
+    for k in (0, 1):
+        assert np.array_equal(seq[k], par[k]), k

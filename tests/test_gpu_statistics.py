@@ -14,7 +14,7 @@ Here the configuration (read by the library's configuration reader) is run on th
 cell, five colonies with seeds 1-5 in one batch, with counter-based draws, to t = 10.  Asserted
 against the shipped run, with bars from the spread of the five seeds:
   - cell counts at t = 2, 4, 6, 8, 10 and the growth rate over the crowded phase t = 7-10;
-  - the cell-area distribution at t = 5 and 10 (L1 distance of the density histograms);
+  - the cell-area distribution at t = 8 and 10 (L1 distance of the density histograms);
   - the mean cell-cycle time of the divisions between t = 1 and 10.
 """
 import os
@@ -123,14 +123,14 @@ def test_cell_area_distribution(runs):
     q = np.load(QUANT, allow_pickle=False)
     w = np.diff(AREA_EDGES)
     centres = (AREA_EDGES[1:] + AREA_EDGES[:-1]) / 2
-    for t in (5, 10):
+    for t in (8, 10):  # earlier, most cells are larger than the bins' 15.5
         i = int(round(t / (gts * dt)))
         mine = np.array([area_hist(fr[i][3]) for fr in frames])
         ref = area_hist(ship[i][3])
         l1 = lambda a, b: float(np.sum(np.abs(a - b) * w))  # noqa: E731  (0 equal, 2 disjoint)
         d_seeds = [l1(m, ref) for m in mine]
         d_pairs = [l1(a, b) for k, a in enumerate(mine) for b in mine[k + 1:]]
-        other = q["cellAreaDistributions"][:, i]
+        other = np.nan_to_num(q["cellAreaDistributions"][:, i])  # an empty histogram is NaN there
         say(f"statistics: area histogram t = {t}: L1 to the shipped run per seed {np.round(d_seeds, 3).tolist()}, "
             f"between seeds max {max(d_pairs):.3f}; other reference runs to the shipped run "
             f"{[round(l1(o, ref), 3) for o in other]}; mean area {np.round([np.sum(m * centres * w) for m in mine], 3).tolist()} "
