@@ -277,6 +277,10 @@ def main():
     if n == N_PARTICLES and os.path.exists(tpath):  # FETCH/WRITE counter runs, copy-calibrated
         tj = json.load(open(tpath))
         traffic, traffic_src = tj["traffic_bytes_per_launch"], tj.get("source")
+    pipeline = None
+    ppath = os.path.join(ROOT, "profiles", "pair_list_1M_pipeline.json")
+    if n == N_PARTICLES and not args.no_list and os.path.exists(ppath):  # SQ / TA / TD counter runs
+        pipeline = json.load(open(ppath))
     out = {
         "metric": "particle-steps/s (plain FP32 step, 3D bulk, 1M particles)",
         "value": value,
@@ -300,7 +304,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_pair32<3>" if args.no_list else "k_pair_list<3>",
                      "kernel_us": pair_us, "kernel_us_window": "steady-state repeats (rebuild passes included)",
-                     "bytes_per_particle": PAIR_BYTES_PER_PARTICLE},
+                     "bytes_per_particle": PAIR_BYTES_PER_PARTICLE,
+                     "pipeline": pipeline},
         "step_hbm_gbs": STEP_BYTES_PER_PARTICLE * value / 1e9,
         "value_definition": "median over the steady-state repeats (each >= 1 s) of particles / (seconds per step); "
                             "steps launched directly in 64-step chunks with HIP events around the pair kernel",
