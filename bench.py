@@ -158,9 +158,6 @@ def copy_calibration(nbytes=256 << 20, reps=3):
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
 
 
-LIST_BINS_PER_CUTOFF = 2
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,19 +196,15 @@ def main():
     md = HipMD(p, box, [sp["capacity"] for sp in col.species])
     if args.bins_per_cutoff:
         md.set_tuning(args.bins_per_cutoff, args.block)
-    elif not args.no_list:
-        # list mode: bins sized (cut-off + skin) / 2 measured fastest (1, 2 and 3 bins per
-        # cut-off at 1M particles, skin 0.3); the stencil autotuner's objective does not apply
-        md.set_tuning(LIST_BINS_PER_CUTOFF, 64)
     md.upload(col)
     md.forces(want=False)
-    tuned = None
-    if not args.bins_per_cutoff and args.no_list:
-        tuned = md.autotune(reps=3)
-        log(f"autotune: {tuned}")
     if not args.no_list:
         md.set_neighbour_list(True, args.skin, args.list_max)
         log(f"neighbour list on: skin {args.skin} sigma, {args.list_max} entries per particle")
+    tuned = None
+    if not args.bins_per_cutoff:  # stencil: bins x block x kernel; list mode: bins per cut-off
+        tuned = md.autotune(reps=3)
+        log(f"autotune: {tuned}")
     ts = md.step(args.warmup, 0)
     # per-family breakdown (every family timed; its event records cost GPU time)
     md.set_timing(True)
@@ -297,8 +290,7 @@ def main():
         "config": {"workload": "3D periodic bulk, 1048576 single discs, phi=0.6, lognormal sigma CV 0.1, "
                                "LJ 2.5 sigma (species 0 pairs) + soft core, dt=1e-4, kT=0.05",
                    "particles": n, "bins": nb, "bin_width": w, "autotuned": tuned,
-                   "neighbour_list": None if args.no_list else {"skin": args.skin, "max_neighbours": args.list_max,
-                                                                 "bins_per_cutoff": LIST_BINS_PER_CUTOFF}},
+                   "neighbour_list": None if args.no_list else {"skin": args.skin, "max_neighbours": args.list_max}},
         "kernels_us": fam,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
@@ -328,8 +320,11 @@ def main():
         for s_ in range(q.nspecies):
             q.species[s_].kT = 0.005
         m2 = HipMD(q, box, [sp["capacity"] for sp in col.species])
-        if not args.no_list:  # the same plain path as the main leg for the fallback steps
-            m2.set_tuning(args.bins_per_cutoff or LIST_BINS_PER_CUTOFF, args.block or 64)
+        # the same plain path as the main leg for the fallback steps
+        if tuned:
+            m2.set_tuning(tuned["bins_per_cutoff"], tuned["block"])
+        elif args.bins_per_cutoff:
+            m2.set_tuning(args.bins_per_cutoff, args.block)
         m2.upload(col)
         if not args.no_list:
             m2.set_neighbour_list(True, args.skin, args.list_max)

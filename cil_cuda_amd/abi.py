@@ -99,7 +99,7 @@ class Run(C.Structure):
     _fields_ = [("nspecies", C.c_int), ("gts", C.c_int), ("frames", C.c_int), ("ncells", C.c_int * MAX_SPECIES),
                 ("species", (C.c_char * 32) * MAX_SPECIES), ("elements", ((C.c_char * 32) * MAX_ELEMS) * MAX_SPECIES),
                 ("kbt", C.c_double), ("seed", C.c_long), ("sigma_max", (C.c_double * MAX_ELEMS) * MAX_SPECIES),
-                ("f_ext", (C.c_double * 3) * MAX_SPECIES)]
+                ("f_ext", (C.c_double * 3) * MAX_SPECIES), ("div_app_record", C.c_int)]
 
 
 class Box(C.Structure):
@@ -214,6 +214,8 @@ def lib():
                                          C.c_int, C.c_double]
     L.cil_hip_h5_write_frame_ctx.argtypes = [vp, vp, C.POINTER(Run), C.c_int, C.c_int, C.c_double]
     L.cil_hip_h5_close.argtypes = [vp]
+    L.cil_hip_set_division_log.argtypes = [vp, C.c_int]
+    L.cil_hip_division_log.argtypes = [vp, C.c_int, PD, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.c_int]
     L.cil_hip_sizeof.argtypes = [C.c_int]
     L.cil_hip_sizeof.restype = C.c_size_t
     _lib = L
@@ -239,6 +241,7 @@ EXPORTS = (
     "cil_hip_config_read", "cil_hip_config_init", "cil_hip_dsfmt_fill", "cil_hip_gsd_create", "cil_hip_gsd_write_chunk",
     "cil_hip_gsd_end_frame", "cil_hip_gsd_close", "cil_hip_gsd_write_frame", "cil_hip_gsd_write_frame_ctx",
     "cil_hip_h5_create", "cil_hip_h5_write_frame", "cil_hip_h5_write_frame_ctx", "cil_hip_h5_close",
+    "cil_hip_set_division_log", "cil_hip_division_log",
     "cil_hip_batch_create", "cil_hip_batch_destroy", "cil_hip_batch_size", "cil_hip_batch_step",
     "cil_hip_batch_autotune", "cil_hip_batch_last_error",
     "cil_hip_mlp_nparams", "cil_hip_mlp_set_params", "cil_hip_mlp_get_params", "cil_hip_mlp_init",

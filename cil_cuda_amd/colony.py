@@ -198,6 +198,23 @@ class HipMD:
     def set_kernel(self, variant):
         _check(self.L, self.ctx, self.L.cil_hip_set_kernel(self.ctx, int(variant)), "set_kernel")
 
+    def set_division_log(self, on=True):
+        """Record each division's new cell (positions right after the division, step)."""
+        _check(self.L, self.ctx, self.L.cil_hip_set_division_log(self.ctx, 1 if on else 0), "set_division_log")
+
+    def division_log(self, s, clear=True):
+        """(positions [n][nelem][dim], steps [n]) of species s's recorded divisions."""
+        n = C.c_int()
+        _check(self.L, self.ctx, self.L.cil_hip_division_log(self.ctx, s, abi.PD(), None, 0, C.byref(n), 0),
+               "division_log")
+        d, ne = self.params.dim, self.params.species[0].nelem
+        pos = np.zeros((max(n.value, 1), ne, d))
+        ts = np.zeros(max(n.value, 1), dtype=np.int32)
+        _check(self.L, self.ctx, self.L.cil_hip_division_log(self.ctx, s, pos.ctypes.data_as(abi.PD),
+                                                             ts.ctypes.data_as(C.POINTER(C.c_int)), n.value,
+                                                             C.byref(n), 1 if clear else 0), "division_log")
+        return pos[:n.value], ts[:n.value]
+
     def set_neighbour_list(self, on=True, skin=0.3, max_neighbours=256):
         _check(self.L, self.ctx, self.L.cil_hip_set_neighbour_list(self.ctx, 1 if on else 0, float(skin),
                                                                    int(max_neighbours)), "set_neighbour_list")

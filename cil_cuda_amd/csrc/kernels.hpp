@@ -82,7 +82,28 @@ struct Dev {
   // and each cell's draws of the current step's division event, [C][8]
   Dsfmt* dsfmt;
   double* ddraw;
+  // division records (the reference's posCellsDiv / cellT0Div, Colony.cpp:319-326), on when
+  // requested: per species region of cap[s] entries from base[s], the new cell's element
+  // positions [C][CIL_MAX_ELEMS][3] and the step, dlog_cnt[s] entries since the last read
+  double* dlog_pos = nullptr;
+  int* dlog_ts = nullptr;
+  int* dlog_cnt = nullptr;
 };
+
+// Records a division of species s (its rank-th this step) whose new cell is slot inew, in
+// the order of the reference's loop (parent order); entries beyond the region are dropped
+// (the host reads and clears the records once per frame).
+__device__ inline void log_division(const StepParams& P, const Dev& S, int s, int rank, int inew, int ts) {
+  if (!S.dlog_pos) return;
+  const int k = S.dlog_cnt[s] + rank;
+  if (k >= P.cap[s]) return;
+  const size_t e = (size_t)P.base[s] + k;
+  for (int q = 0; q < P.nE; ++q)
+    for (int d = 0; d < 3; ++d)
+      S.dlog_pos[(e * CIL_MAX_ELEMS + q) * 3 + d] =
+          d < P.box.dim ? S.pos[(size_t)d * S.P + (size_t)inew * P.nE + q] : 0.0;
+  S.dlog_ts[e] = ts;
+}
 
 // ---------------------------------------------------------------------------------------
 // Small helpers

@@ -227,6 +227,7 @@ typedef struct {
   /* parameters the step does not use, kept for the run's HDF5 parameters */
   double sigma_max[CIL_MAX_SPECIES][CIL_MAX_ELEMS]; /* PAIRFORCES.SC.SIGMA_MAX (0 for other laws) */
   double f_ext[CIL_MAX_SPECIES][3];      /* EXTFORCE (the reference adds 0 for it, CIL.cu:55-66) */
+  int div_app_record;                    /* OUTPUT.DIV_APP_RECORD: the HDF5 file keeps /div_app */
 } cil_run;
 
 typedef struct cil_ctx cil_ctx;
@@ -294,8 +295,20 @@ int cil_hip_set_tuning(cil_ctx* ctx, int bins_per_cutoff, int block);
  * kernel (default), 1 = generic kernel. */
 int cil_hip_set_kernel(cil_ctx* ctx, int variant);
 
+/* Division records (project addition; the reference's posCellsDiv / cellT0Div, written to the
+ * HDF5 group /div_app, Colony.cpp:319-344, OutputWriter.cpp:439-470).  on = 1 records, for each
+ * division, the new cell's element positions right after the division and its step, per
+ * species in the order the reference's loop divides (parent order), up to the species'
+ * capacity between reads; on = 0 stops and frees.  Enable before creating a batch that holds
+ * the context.  cil_hip_division_log copies species s's records (pos [n][nelem][dim], ts [n];
+ * either may be NULL) and clears them when clear != 0; *n = records held (only cap copied). */
+int cil_hip_set_division_log(cil_ctx* ctx, int on);
+int cil_hip_division_log(cil_ctx* ctx, int s, double* pos, int* ts, int cap, int* n, int clear);
+
 /* Autotuner (project addition): times `reps` force passes for every bins-per-cut-off /
- * pair-block / kernel-variant candidate, keeps the fastest and reports it.  Needs forces
+ * pair-block / kernel-variant candidate, keeps the fastest and reports it.  In neighbour-list
+ * mode it chooses the bins per cut-off (1-3) that minimise a pass without a rebuild plus a
+ * rebuild pass spread over 45 steps (block and variant are reported unchanged).  Needs forces
  * computed by cil_hip_forces or a step (the state is not advanced). */
 int cil_hip_autotune(cil_ctx* ctx, int reps, int* best_bins, int* best_block, int* best_variant, double* best_us);
 
@@ -316,7 +329,7 @@ int cil_hip_set_timing(cil_ctx* ctx, int on);
  * and re-sorts / rebuilds when a particle has moved more than skin / 2 since the build, when
  * cells divide or are removed, or after an upload.  The pair set equals the stencil kernel's
  * (FP32 summation order differs).  on = 0 returns to the stencil kernels.  Bins are sized for
- * cut-off + skin while it is on; cil_hip_autotune requires it off. */
+ * cut-off + skin while it is on. */
 int cil_hip_set_neighbour_list(cil_ctx* ctx, int on, double skin, int max_neighbours);
 
 /* ---------------------------------------------------------------------------------------
@@ -385,8 +398,11 @@ int cil_hip_gsd_write_frame_ctx(cil_ctx* ctx, cil_gsd* g, const cil_run* run, in
  * file (the reference's `.Last.h5`).  cil_hip_h5_write_frame adds /snapshots/t_<id> (attribute
  * ts), or for id < 0 writes at the root, R, Vel, SigmaCore, F, FD, FR, RHO, Theta, PR, State,
  * T0 and CpuTime per species (a subgroup per species when there are several), reals as 32-bit
- * floats.  FD and FR (the DPD forces) are zero.  The division / apoptosis records (/div_app) are
- * not written.
+ * floats.  FD and FR (the DPD forces) are zero.  With run->div_app_record the file also holds
+ * /div_app/DIV and /div_app/APP, and cil_hip_h5_write_frame_ctx writes each frame's division
+ * records (d_<id>: R, T) from the context's division log (cil_hip_set_division_log must be on;
+ * cil_hip_h5_write_frame, from host views, writes none).  There is no apoptosis: APP stays
+ * empty.
  * The HDF5 library is loaded at the first create ($CIL_HDF5_LIB, libhdf5.so, or
  * /opt/conda/lib/libhdf5.so); without it the calls return CIL_E_IO. */
 typedef struct cil_h5 cil_h5;

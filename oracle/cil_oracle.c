@@ -284,6 +284,11 @@ struct or_ws {
   int com_cap;
   int phase; /* 1 while the Heun corrector's forces are evaluated (stream offset ST_PHASE) */
   int sample; /* threaded gather: pair forces only for cells whose UID is a multiple of sample (0/1: all) */
+  /* division records (posCellsDiv / cellT0Div, Colony.cpp:319-326; CIL.cu:396-399, 542-545) */
+  int log_on;
+  int log_n[CIL_MAX_SPECIES], log_cap[CIL_MAX_SPECIES];
+  double* log_pos[CIL_MAX_SPECIES];
+  int* log_ts[CIL_MAX_SPECIES];
   char err[256];
 };
 
@@ -332,6 +337,13 @@ void or_destroy(or_ws* w) {
   free(w->list);
   free(w->ihead);
   free(w->com);
+  {
+    int s;
+    for (s = 0; s < CIL_MAX_SPECIES; s++) {
+      free(w->log_pos[s]);
+      free(w->log_ts[s]);
+    }
+  }
   free(w);
 }
 
@@ -342,6 +354,36 @@ void or_set_rng(or_ws* w, int mode, uint32_t seed) {
 double or_rng_draw(or_ws* w, int kind) { return kind == 1 ? or_dsfmt_normal(&w->g) : or_dsfmt_close_open(&w->g); }
 void or_set_traversal(or_ws* w, int mode) { w->traversal = mode; }
 void or_set_sample(or_ws* w, int every) { w->sample = every; }
+void or_set_division_log(or_ws* w, int on) { w->log_on = on; }
+
+/* Records the new cell inew of species s right after its division, as the reference appends
+ * posCellsDiv[n] = pos[inew] and cellT0Div[n] = ts. */
+static void log_division(or_ws* w, int s, const cil_species_view* V, int inew, int ts) {
+  const int per = w->nE * w->box.dim;
+  int k;
+  if (!w->log_on) return;
+  if (w->log_n[s] == w->log_cap[s]) {
+    int cap = w->log_cap[s] ? 2 * w->log_cap[s] : 64;
+    w->log_pos[s] = (double*)realloc(w->log_pos[s], sizeof(double) * (size_t)cap * per);
+    w->log_ts[s] = (int*)realloc(w->log_ts[s], sizeof(int) * (size_t)cap);
+    w->log_cap[s] = cap;
+  }
+  for (k = 0; k < per; k++) w->log_pos[s][(size_t)w->log_n[s] * per + k] = V->pos[(size_t)inew * per + k];
+  w->log_ts[s][w->log_n[s]] = ts;
+  w->log_n[s]++;
+}
+
+int or_division_log(or_ws* w, int s, double* pos, int* ts, int cap, int* n, int clear) {
+  const int per = w->nE * w->box.dim;
+  int m;
+  if (s < 0 || s >= CIL_MAX_SPECIES || !n) return CIL_E_INVALID;
+  *n = w->log_n[s];
+  m = w->log_n[s] < cap ? w->log_n[s] : cap;
+  if (m > 0 && pos) memcpy(pos, w->log_pos[s], sizeof(double) * (size_t)m * per);
+  if (m > 0 && ts) memcpy(ts, w->log_ts[s], sizeof(int) * (size_t)m);
+  if (clear) w->log_n[s] = 0;
+  return 0;
+}
 int or_set_threads(or_ws* w, int n) {
   w->nthreads = n < 1 ? 1 : n;
   return w->nthreads;
@@ -1250,6 +1292,7 @@ static int divisions_theta(or_ws* w, cil_species_view* v, int nspec, int ts) {
           box_update_pos_dr(&w->box, &V->pos[(size_t)inew * D], dr);
           for (d = 0; d < D; d++) dr[d] *= -1;
           box_update_pos_dr(&w->box, &V->pos[(size_t)i * D], dr);
+          log_division(w, s, V, inew, ts);
           /* sigma halving, CIL.cu:546-570 */
           if (D == 2) newSig = lMom / sqrt(2.0);
           else {
@@ -1331,6 +1374,7 @@ static int divisions_linear(or_ws* w, cil_species_view* v, int nspec, int ts) {
           dx[d] = -P->l_birth * uh[d];
         }
         box_update_pos_dr(&w->box, &V->pos[((size_t)inew * 2) * D], dx);
+        log_division(w, s, V, inew, ts);
         /* daughter A (parent slot): back = b, front = b + l_birth*uh */
         for (d = 0; d < D; d++) {
           f[d] = b[d];
@@ -1437,6 +1481,7 @@ static int divisions_binary(or_ws* w, cil_species_view* v, int nspec, int ts) {
           for (d = 0; d < D; d++) nb[d] = b[d];
           binary_reorient(w, b, f, P->division_r0, key, ts, 0);   /* mother keeps the front */
           binary_reorient(w, nf, nb, P->division_r0, key, ts, 2); /* daughter the old back */
+          log_division(w, s, V, inew, ts);
           V->t0[inew] = ts;
           V->t0[i] = ts;
           V->tswitch[inew] = ts + random_span(w, &P->migration_tau, key, ts, 4, 0);

@@ -61,6 +61,11 @@ void or_set_traversal(or_ws* w, int mode);
 /* threaded gather (threads > 1, counter RNG): pair forces of the cells whose UID is a multiple of
  * `every` only, the other cells keep their intra forces (a sample of a large colony) */
 void or_set_sample(or_ws* w, int every);
+/* division records: on = 1 appends, per division, the new cell's element positions right after
+ * the division and the step; or_division_log copies species s's records (pos [n][nelem][dim])
+ * and clears them when clear != 0 */
+void or_set_division_log(or_ws* w, int on);
+int or_division_log(or_ws* w, int s, double* pos, int* ts, int cap, int* n, int clear);
 int or_set_threads(or_ws* w, int nthreads); /* OpenMP gather variant when > 1 */
 
 /* calculForcesPressure, CIL.cu:999-1007 (draws noise when enabled). */

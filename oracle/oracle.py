@@ -38,6 +38,8 @@ def lib(fast: bool = False):
     L.or_set_traversal.argtypes = [vp, C.c_int]
     L.or_set_threads.argtypes = [vp, C.c_int]
     L.or_set_sample.argtypes = [vp, C.c_int]
+    L.or_set_division_log.argtypes = [vp, C.c_int]
+    L.or_division_log.argtypes = [vp, C.c_int, abi.PD, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.c_int]
     L.or_forces.argtypes = [vp, PV, C.c_int, C.c_int]
     L.or_step.argtypes = [vp, PV, C.c_int, C.c_int, abi.PI]
     L.or_update.argtypes = [vp, PV, C.c_int, C.c_int]
@@ -101,6 +103,19 @@ class Oracle:
     def set_sample(self, every):
         """Threaded gather only: pair forces of every `every`-th cell (by UID) only."""
         self.L.or_set_sample(self.w, int(every))
+
+    def set_division_log(self, on=True):
+        self.L.or_set_division_log(self.w, 1 if on else 0)
+
+    def division_log(self, s, dim, nelem, clear=True):
+        """(positions [n][nelem][dim], steps [n]) of species s's recorded divisions."""
+        n = C.c_int()
+        self.L.or_division_log(self.w, s, abi.PD(), None, 0, C.byref(n), 0)
+        pos = np.zeros((max(n.value, 1), nelem, dim))
+        ts = np.zeros(max(n.value, 1), dtype=np.int32)
+        self.L.or_division_log(self.w, s, pos.ctypes.data_as(abi.PD), ts.ctypes.data_as(C.POINTER(C.c_int)), n.value,
+                               C.byref(n), 1 if clear else 0)
+        return pos[:n.value], ts[:n.value]
 
     def set_rng(self, mode, seed=0):
         self.L.or_set_rng(self.w, mode, seed)

@@ -132,6 +132,45 @@ def test_batch_autotune_keeps_colonies_consistent(gpu):
     assert ts == [12, 12]
 
 
+@pytest.mark.parametrize("model", ["growth", "theta"])
+def test_batched_divisions_match_the_oracle(gpu, model):
+    """Colonies with division stepped in one batch against the oracle stepped alone: the same
+    division events (cell counts and birth steps) and positions within 1e-9 sigma (FP64)."""
+    from oracle import oracle as O
+
+    if model == "growth":
+        builders = [configs.channel_2d_hz(length=length, seed=60 + k, kT=kT, tau=tau, capacity=512)
+                    for k, (tau, kT, length) in enumerate(((0.0015, 0.001, 64.0), (0.002, 0.005, 96.0)))]
+        nsteps = 240
+    else:
+        builders = [configs.theta_2d_hz(n_cells=n, seed=70 + n) for n in (12, 20)]
+        nsteps = 64
+    members = [_make(p, box, col) for p, box, col in builders]
+    batch = HipBatch(members)
+    assert batch.step(nsteps, [0] * len(members)) == [nsteps] * len(members)
+    got = [_snapshot(m, col) for m, (p, box, col) in zip(members, builders)]
+    batch.close()
+    for m in members:
+        m.close()
+    sys.path.insert(0, __file__.rsplit("/", 1)[0])
+    from test_gpu_parity import _max_disp_err
+
+    for k, (p, box, col) in enumerate(builders):
+        ref = col.copy()
+        orc = O.Oracle(p, box, traversal=1)
+        orc.forces(ref, 0)
+        orc.step(ref, nsteps, 0)
+        orc.close()
+        a, b = got[k], ref
+        assert a.ncells() == b.ncells(), (k, a.ncells(), b.ncells())
+        n = a.ncells()[0]
+        assert np.array_equal(a.species[0]["t0"][:n], b.species[0]["t0"][:n])
+        err = _max_disp_err(b, a, box)
+        say(f"{model} colony {k}: {n} cells ({col.ncells()[0]} at the start), {err:.2e} sigma from the oracle")
+        assert err <= 1e-9
+    assert any(g.ncells()[0] > c.ncells()[0] for g, (_, _, c) in zip(got, builders))  # divisions happened
+
+
 def test_batch_rejects_mixed_launch_shapes(gpu):
     a = configs.plain_2d_sc(n_cells=40, L=12.0, seed=1)
     b = configs.plain_2d_sc(n_cells=40, L=12.0, seed=2, precision=abi.PREC_FP32)

@@ -314,3 +314,31 @@ def test_two_list_contexts_step_concurrently(gpu):
     assert set(par) == {0, 1}
     for k in (0, 1):
         assert np.array_equal(seq[k], par[k]), k
+
+
+def test_list_autotune_keeps_the_state(gpu):
+    """The list-mode autotuner times rebuild and plain passes at 1-3 bins per cut-off at the
+    current positions: the state is not advanced, the forces it leaves are those of a fresh
+    context with the chosen bins (bit for bit), and stepping continues."""
+    p, box, col = _relaxed(8192, 4)
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    md.set_neighbour_list(True, 0.3, 256)
+    md.forces(want=False)
+    before = md.download(col.copy())
+    best = md.autotune(reps=2)
+    say(f"list autotune: {best}")
+    assert best["bins_per_cutoff"] in (1, 2, 3) and best["force_pass_us"] > 0.0
+    after = md.download(col.copy())
+    assert np.array_equal(before.flat("pos"), after.flat("pos"))
+    fresh = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    fresh.set_tuning(best["bins_per_cutoff"], 64)
+    fresh.upload(before)
+    fresh.set_neighbour_list(True, 0.3, 256)
+    fresh.forces(want=False)
+    ref = fresh.download(col.copy())
+    fresh.close()
+    assert np.array_equal(after.flat("frc"), ref.flat("frc"))
+    say(f"list autotune: forces against FP64, worst {_force_bar(p, box, after, after.flat('frc').reshape(-1, 3)):.3f}")
+    assert md.step(50, 0) == 50
+    md.close()
