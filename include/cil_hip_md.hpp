@@ -8,12 +8,19 @@
 //   cil::hip::CellColony     per-species arrays in the reference layout, addCell/removeCell
 //   cil::hip::Session        owns one device context (upload, step, download)
 //   cil::hip::EulerHIP       `void (*)(int& ts)`-compatible step on the bound session
+//                            (StochasticHeunHIP: the same seam for STOCHASTIC_HEUN runs)
 //   cil::hip::Batch          many sessions stepped by one launch per kernel
+//   cil::hip::RunConfig      a reference JSON run file read into parameters, box and run
+//                            settings, and its initial colony (readMD/readSpecs/..., the INIT
+//                            block and initializeCellStatus on the reference's stream)
+//   cil::hip::GsdFile, H5File  frames in the reference's GSD and HDF5 formats from a session
 //
 // Errors surface as std::runtime_error, the reference's run_assert convention
 // (Utils.hpp:99-109).  Host arrays are never cached by the device side.
 #pragma once
 
+#include <fstream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -153,11 +160,82 @@ class Session {
     run_assert(cil_hip_set_neighbour_list(ctx_, on ? 1 : 0, skin, max_neighbours), ctx_,
                "cil_hip_set_neighbour_list");
   }
+  // Division records for the HDF5 /div_app group (project addition; cil_hip_set_division_log)
+  void setDivisionLog(bool on) {
+    run_assert(cil_hip_set_division_log(ctx_, on ? 1 : 0), ctx_, "cil_hip_set_division_log");
+  }
+  CellColony& colony() { return col_; }
   cil_ctx* ctx() { return ctx_; }
 
  private:
   CellColony& col_;
   cil_ctx* ctx_ = nullptr;
+};
+
+// A reference run file (project addition over cil_hip_config_read / cil_hip_config_init).
+struct RunConfig {
+  cil_params params{};
+  cil_box box{};
+  cil_run run{};
+  std::string text;
+
+  static RunConfig read(const std::string& path) {
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("RunConfig: cannot open " + path);
+    std::stringstream ss;
+    ss << in.rdbuf();
+    RunConfig c;
+    c.text = ss.str();
+    char err[512] = {0};
+    if (cil_hip_config_read(c.text.c_str(), &c.params, &c.box, &c.run, err, sizeof err) != CIL_OK)
+      throw std::runtime_error(std::string("RunConfig: ") + err);
+    return c;
+  }
+  // The INIT cells into `col` (capacity at least run.ncells[s] per species); sets
+  // params.rng_offset to the draws initializeCellStatus consumed.
+  void initColony(CellColony& col) {
+    auto v = col.views();
+    char err[512] = {0};
+    if (cil_hip_config_init(text.c_str(), &params, v.data(), (int)v.size(), err, sizeof err) != CIL_OK)
+      throw std::runtime_error(std::string("RunConfig::initColony: ") + err);
+    col.pull_counts(v);
+  }
+};
+
+// outputGSDFrame's file (project addition over cil_hip_gsd_*).
+class GsdFile {
+ public:
+  explicit GsdFile(const std::string& path) {
+    if (cil_hip_gsd_create(path.c_str(), "CIL", "hoomd", (1u << 16) | 1u, &g_) != CIL_OK)
+      throw std::runtime_error("GsdFile: cannot create " + path);
+  }
+  ~GsdFile() { cil_hip_gsd_close(g_); }
+  GsdFile(const GsdFile&) = delete;
+  GsdFile& operator=(const GsdFile&) = delete;
+  void frame(Session& s, const cil_run* run, int ts) {
+    run_assert(cil_hip_gsd_write_frame_ctx(s.ctx(), g_, run, ts), s.ctx(), "cil_hip_gsd_write_frame_ctx");
+  }
+
+ private:
+  cil_gsd* g_ = nullptr;
+};
+
+// initHDF5Output / outputHDF5Frame's file (project addition over cil_hip_h5_*).
+class H5File {
+ public:
+  H5File(const std::string& path, const cil_params& p, const cil_box& box, const cil_run* run) {
+    if (cil_hip_h5_create(path.c_str(), &p, &box, run, &f_) != CIL_OK)
+      throw std::runtime_error("H5File: cannot create " + path + " (is libhdf5.so loadable?)");
+  }
+  ~H5File() { cil_hip_h5_close(f_); }
+  H5File(const H5File&) = delete;
+  H5File& operator=(const H5File&) = delete;
+  void frame(Session& s, const cil_run* run, int id, int ts, double cpu_time = 0.0) {
+    run_assert(cil_hip_h5_write_frame_ctx(s.ctx(), f_, run, id, ts, cpu_time), s.ctx(), "cil_hip_h5_write_frame_ctx");
+  }
+
+ private:
+  cil_h5* f_ = nullptr;
 };
 
 // Many sessions advanced together, one launch per kernel for all colonies
@@ -197,6 +275,9 @@ inline void EulerHIP(int& ts) {
   if (!s) throw std::runtime_error("EulerHIP: no session bound");
   s->steps(1, ts);
 }
+// StochasticHeun (CIL.cu:1028-1049): the session's parameters select the integrator
+// (cil_params.integrator = CIL_INT_HEUN), so the seam is the same one step.
+inline void StochasticHeunHIP(int& ts) { EulerHIP(ts); }
 
 }  // namespace hip
 }  // namespace cil

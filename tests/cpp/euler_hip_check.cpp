@@ -3,6 +3,7 @@
 // with the CPU oracle on the same input.  Exit codes: 0 ok, 1 mismatch, 3 no GPU device.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 
@@ -139,8 +140,43 @@ int main() {
           err32 = std::fmax(err32, std::fabs(x));
         }
     std::printf("FP32 neighbour-list session: max position difference %.3e\n", err32);
+    // the reference's seeded run from its own JSON file: two frames through the C++ layer,
+    // written in the reference's GSD format from the device, against the oracle's frames
+    bool run_ok = true;
+    if (const char* cfg = std::getenv("CIL_CHECK_CONFIG")) {
+      auto rc = cil::hip::RunConfig::read(cfg);
+      cil::hip::CellColony rg(rc.params.dim, rc.params.species[0].nelem, {64}), ro(rc.params.dim, 1, {64});
+      rc.initColony(rg);
+      ro = rg;
+      cil::hip::Session s4(rc.params, rc.box, rg);
+      s4.upload();
+      s4.calculForcesPressure();
+      or_ws* w4 = or_create(&rc.params, &rc.box);
+      or_set_traversal(w4, 1);
+      auto v4 = ro.views();
+      int tsg = 0, tso4 = 0;
+      or_forces(w4, v4.data(), 1, 0);
+      {
+        cil::hip::GsdFile gsd(std::string(std::getenv("CIL_CHECK_OUT") ? std::getenv("CIL_CHECK_OUT") : "/tmp") +
+                              "/check.gsd");
+        for (int f = 0; f < 2; ++f) {
+          s4.steps(rc.run.gts, tsg);
+          gsd.frame(s4, &rc.run, tsg);
+          or_step(w4, v4.data(), 1, rc.run.gts, &tso4);
+        }
+      }
+      ro.pull_counts(v4);
+      s4.download();
+      double e4 = 0.0;
+      for (int c = 0; c < rg.getNumCells(0); ++c)
+        for (int d = 0; d < 2; ++d) e4 = std::fmax(e4, std::fabs(rg.pos(0, c, 0, d) - ro.pos(0, c, 0, d)));
+      or_destroy(w4);
+      run_ok = tsg == tso4 && rg.getNumCells(0) == ro.getNumCells(0) && e4 <= 1e-9;
+      std::printf("seeded run from its configuration: %d cells after %d steps, %.3e from the oracle\n",
+                  rg.getNumCells(0), tsg, e4);
+    }
     return (ts == nsteps && err <= 1e-9 && same && differs && tsb[0] == nsteps && tsb[1] == nsteps &&
-            ts3 == nsteps && err32 <= 1e-5)
+            ts3 == nsteps && err32 <= 1e-5 && run_ok)
                ? 0
                : 1;
   } catch (const std::runtime_error& e) {

@@ -1,6 +1,7 @@
 """The C++ host layer (include/cil_hip_md.hpp: CellColony, Session, EulerHIP as an
-MDIntegrator) builds against the C ABI and, on a GPU, matches the oracle over 200 steps
-(FP64 within 1e-9 sigma; FP32 over neighbour lists within 1e-5 sigma)."""
+MDIntegrator, RunConfig, GsdFile) builds against the C ABI and, on a GPU, matches the oracle
+over 200 steps (FP64 within 1e-9 sigma; FP32 over neighbour lists within 1e-5 sigma) and over
+two frames of the reference's seeded run read from its JSON file, written as GSD frames."""
 import os
 import subprocess
 
@@ -30,8 +31,11 @@ def test_cpp_host_layer_builds_and_reports_missing_device():
 
 
 @pytest.mark.gpu
-def test_cpp_euler_hip_matches_oracle(gpu):
+def test_cpp_euler_hip_matches_oracle(gpu, tmp_path):
     exe = EXE if os.path.exists(EXE) else _build()
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    env = dict(os.environ, CIL_CHECK_CONFIG=os.path.join(ROOT, "tests", "golden", "configs",
+                                                         "z0.001t3.16_v2.RunParams.json"),
+               CIL_CHECK_OUT=str(tmp_path))
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120, env=env)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
