@@ -135,6 +135,8 @@ class SurrogateStats(C.Structure):
 
 MLP_NPARAM = 26243
 MLP_NFEAT = 72
+MLP_OUT_SCALE = 0.002   # length units per unit network output (the drift correction)
+SURR_BASE_SCALE = 0.01  # length units per unit of features 66-71 (one-pass drift, noise sum)
 
 
 class CilError(RuntimeError):

@@ -1,12 +1,15 @@
 """Drift-surrogate workflow over the C ABI: training from plain trajectories and the noise
-floor of any predictor that sees only the macro-step's start state.
+floor of a predictor that sees only the macro-step's start state.
 
-The surrogate replaces 8 Euler steps by x += D(x0) + (exact sum of the 8 noise increments).
-The plain method's deterministic displacement over those 8 steps, sum_k dt mu F(x_k), depends
-on the noise through the intermediate positions x_k, so no function of x0 alone can match it
-better than its spread over noise realisations.  `noise_floor` measures that spread by running
-the same start state with two noise seeds: E|D_a - D_b|^2 / 2 is the conditional variance, and
-its square root is the smallest RMS error any x0-only predictor can reach.
+The surrogate replaces 8 Euler steps by x += b + C + z: b = 8 dt mu F(x0) the one-pass drift
+(one force pass), z the exact sum of the 8 noise increments, and C the network's correction,
+predicted from the neighbourhood of x0, b and z.  The plain method's deterministic displacement
+over those 8 steps, sum_k dt mu F(x_k), depends on the noise through the intermediate positions
+x_k, so no function of x0 alone can match it better than its spread over noise realisations.
+`noise_floor` measures that spread by running the same start state with two noise seeds:
+E|D_a - D_b|^2 / 2 is the conditional variance, and its square root is the smallest RMS error
+an x0-only predictor can reach.  The network also sees z, which carries the part of that spread
+that is linear in the noise sum, so it can go below this floor.
 """
 from __future__ import annotations
 
@@ -34,7 +37,9 @@ def train(md: HipMD, ts: int, n_samples: int, nmacro: int, steps: int, batch: in
 
 def noise_floor(params: abi.Params, box: abi.Box, colony, ts: int = 0, seeds=(101, 202)):
     """RMS (per particle, length units) of the deterministic 8-step displacement's spread over
-    noise realisations from the same start state, and the RMS of that displacement itself."""
+    noise realisations from the same start state, and the RMS of that displacement itself.
+    (The training targets are that displacement minus the one-pass drift b, the same for both
+    seeds; b is added back from the features, BF16-rounded.)"""
     T = []
     for seed in seeds:
         q = abi.Params.from_buffer_copy(params)
@@ -44,7 +49,8 @@ def noise_floor(params: abi.Params, box: abi.Box, colony, ts: int = 0, seeds=(10
             md.upload(colony)
             md.forces(want=False)
             md.surrogate_collect(1, 1, colony.total_cells(), ts)
-            T.append(md.surrogate_get_samples(with_x=False)[1].astype(np.float64))
+            X, t = md.surrogate_get_samples()
+            T.append(t.astype(np.float64) + abi.SURR_BASE_SCALE * X[:, 66:69].astype(np.float64))
         finally:
             md.close()
     d = T[0] - T[1]

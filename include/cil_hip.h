@@ -450,11 +450,12 @@ const char* cil_hip_batch_last_error(cil_batch* batch);
  * Drift surrogate (project addition; no counterpart in the reference).
  *
  * A per-particle MLP (72 -> 128 -> 128 -> 3, SiLU, BF16 weights and activations on MFMA,
- * FP32 accumulation) predicts the deterministic displacement over 8 Euler steps from the
- * 16 nearest neighbours, each given as (u m, 20 g): u the unit vector towards it, m = sigma_j
- * (species 0) or sigma_j + 4 (species 1), g = r / sigma_bar - 1; then own sigma, own species
- * and 6 zeros (72 features).  A macro-step adds that drift (capped at 0.05 sigma per particle,
- * a trust bound) plus the exact sum of the 8 steps' FDT noise increments.  Supported: 3D
+ * FP32 accumulation) predicts the correction to the one-pass drift b = 8 dt mu F(x0) of a
+ * macro-step of 8 Euler steps.  Features: the 16 nearest neighbours, each given as (u m, 20 g):
+ * u the unit vector towards it, m = sigma_j (species 0) or sigma_j + 4 (species 1),
+ * g = r / sigma_bar - 1; own sigma and species; b and the sum z of the 8 steps' FDT noise
+ * increments, both / 0.01 (72 features).  A macro-step moves x0 + b + 0.002 y + z, the
+ * correction 0.002 y capped at 0.05 sigma per particle (a trust bound).  Supported: 3D
  * periodic single-disc colonies in FP32 force mode, no
  * division, walls, motility, pressure friction, drag or active noise; other set-ups return
  * CIL_E_INVALID.  The kNN search visits shells of bins around each particle until the 16th
@@ -479,20 +480,22 @@ int cil_hip_mlp_get_params(cil_ctx* ctx, float* params, size_t n);
 /* Weights and biases uniform in (-1/sqrt(fan_in), 1/sqrt(fan_in)), from Philox keyed by seed;
  * resets the Adam state. */
 int cil_hip_mlp_init(cil_ctx* ctx, uint64_t seed);
-/* Y (n, 3) = drift in length units for features X (n, 72), rounded to BF16 on entry. */
+/* Y (n, 3) = 0.002 x network output (the drift correction, length units) for features
+ * X (n, 72), rounded to BF16 on entry. */
 int cil_hip_mlp_forward(cil_ctx* ctx, const float* X, size_t n, float* Y);
-/* Gradient of the loss mean((y - T / 0.01)^2) / 2 over the n x 3 outputs (y = network output
- * before the 0.01 output scale; T in length units), no update.  n <= 65536. */
+/* Gradient of the loss mean((y - T / 0.002)^2) / 2 over the n x 3 outputs (y = network output
+ * before the 0.002 output scale; T in length units), no update.  n <= 65536. */
 int cil_hip_mlp_grad(cil_ctx* ctx, const float* X, const float* T, size_t n, float* grad, float* loss);
-/* Features of the current state, rows in UID order (species-major). */
+/* Features of the current state at the context's step counter (the last ts an entry point
+ * returned), rows in UID order (species-major); computes the forces if none are held. */
 int cil_hip_knn_features(cil_ctx* ctx, float* F, size_t rows);
 /* Runs nmacro x 8 plain Euler steps from the current state (ts advances by 8 per macro-step)
  * and appends, for every stride-th sorted particle of each macro-step start (rotating phase),
- * its features and its 8-step displacement minus the exact noise sum to the device training
- * set (capacity cap samples). */
+ * its features and its 8-step displacement minus the noise sum z and the one-pass drift b
+ * (the correction the network learns) to the device training set (capacity cap samples). */
 int cil_hip_surrogate_collect(cil_ctx* ctx, int nmacro, int stride, long cap, int* ts, long* nsamples);
 /* Replaces the training set with n host samples: X (n, 72) rounded to BF16, T (n, 3) drift
- * in length units. */
+ * correction in length units. */
 int cil_hip_surrogate_set_samples(cil_ctx* ctx, const float* X, const float* T, long n);
 /* Copies up to cap training samples back (X as float, T in length units); *n = samples held. */
 int cil_hip_surrogate_get_samples(cil_ctx* ctx, float* X, float* T, long cap, long* n);

@@ -5,10 +5,13 @@ Restates what the HIP surrogate kernels compute, so they can be checked:
     ties broken by particle index) in ascending distance, each as (u_x m, u_y m, u_z m, 20 g)
     with u = d / r the unit vector towards the neighbour, m = sigma_j for species 0 and
     sigma_j + 4 for species 1, g = r / sigma_bar - 1 (sigma_bar = (sigma_i + sigma_j) / 2);
-    then sigma_i, species_i, the cell axis (3 zeros for single discs) and 3 zero pads -> 72
-    features.  (A bijective re-expression of (dx, dy, dz, sigma_j, species_j) that BF16
-    stores to ~0.4 % relative.)
-  - MLP 72 -> 128 -> 128 -> 3, SiLU after the hidden layers, output scaled by OUT_SCALE;
+    then sigma_i, species_i (features 64, 65); the macro-step's one-pass drift
+    b = nsub dt mu F(x0) (features 66-68) and the sum z of the FDT noise displacements of its
+    nsub steps (69-71), both in units of BASE_SCALE -> 72 features.  (The neighbour part is a
+    bijective re-expression of (dx, dy, dz, sigma_j, species_j) that BF16 stores to ~0.4 %
+    relative.)
+  - MLP 72 -> 128 -> 128 -> 3, SiLU after the hidden layers, output scaled by OUT_SCALE: the
+    correction to b, so the macro-step moves x0 + b + OUT_SCALE y + z;
   - loss: mean over samples and components of (y - t)^2 / 2 on the scaled target;
   - backward and an Adam step (Kingma & Ba 2015).
 The surrogate has no reference counterpart: parity of these functions is pinned by finite
@@ -22,7 +25,10 @@ NNB = 16
 NFEAT = 72
 HIDDEN = 128
 NOUT = 3
-OUT_SCALE = 0.01
+OUT_SCALE = 0.002
+BASE_SCALE = 0.01
+NSUB = 8
+ST_FDT = 2
 
 
 def knn_features(pos, sigma, species, box_len, k=NNB):
@@ -47,6 +53,27 @@ def knn_features(pos, sigma, species, box_len, k=NNB):
         feats[i, 64] = sigma[i]
         feats[i, 65] = species[i]
     return feats
+
+
+def macro_inputs(frc, mobility_dt, noise, nsub=NSUB):
+    """Features 66-71 of particles with forces frc (n, 3), dt / (zeta area) mobility_dt (n,)
+    and noise sums (n, 3)."""
+    b = nsub * mobility_dt[:, None] * frc
+    return np.concatenate([b, noise], axis=1) / BASE_SCALE
+
+
+def fdt_noise_sum(seed, keys, amp, ts0, nsub=NSUB, dim=3):
+    """Sum over steps ts0..ts0+nsub-1 of the FDT displacements amp * z of particles with
+    counter keys `keys` ((cell key << 1) | disc), z the first `dim` of the four normals of the
+    step's draws (sub 0 and 1), as the oracle's step draws them (cil_oracle.c, FDT noise)."""
+    from oracle import oracle as O
+
+    out = np.zeros((len(keys), dim))
+    for i, k in enumerate(keys):
+        for t in range(ts0, ts0 + nsub):
+            z = O.philox_normal2(seed, k, t, ST_FDT, 0) + O.philox_normal2(seed, k, t, ST_FDT, 1)
+            out[i] += amp[i] * np.array(z[:dim])
+    return out
 
 
 SPECIES1_SHIFT = 4.0

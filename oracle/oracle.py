@@ -199,6 +199,14 @@ def philox(ctr, key):
     return [o[i] for i in range(4)]
 
 
+def philox_normal2(seed, key, ctr, stream, sub):
+    """The two standard normals of one counter-based draw (the FDT noise of a step takes sub 0
+    and 1 of (seed, particle key, step, stream 2))."""
+    z = (C.c_double * 2)()
+    lib().or_philox_normal2(int(seed), int(key), int(ctr), int(stream), int(sub), z)
+    return z[0], z[1]
+
+
 def pair_f0(params, dim, ti, tj, si, sj, r2):
     f = C.c_double()
     hit = lib().or_pair_f0(C.byref(params), dim, ti, tj, si, sj, r2, C.byref(f))

@@ -8,11 +8,13 @@ weights and inputs):
   - Adam: one step moves every parameter whose reference gradient is not tiny by lr
     (first Adam step with bias correction), to 1e-3 * lr, against the gradient's sign for
     >= 99.9 % of them;
-  - kNN features: equal to the oracle's rows rounded to BF16 (one BF16 ulp) for every
+  - kNN features 0-65: equal to the oracle's rows rounded to BF16 (one BF16 ulp) for every
     particle without a near-tie (relative gap < 1e-4) among its 17 nearest distances;
+  - features 66-71 (one-pass drift from the oracle's FP64 forces, noise sum from the oracle's
+    draws): one BF16 ulp, plus 1e-3 of the drift's RMS for the FP32 forces;
   - training on plain trajectories (kT = 0.005): the trained network's 8-step displacement
-    error is at least 10 % below the zero-drift error (what the network gains over the
-    method without it); printed next to one Euler step of 8 dt.
+    error is below that of one Euler step of 8 dt (the network's starting point: its
+    correction added to that step) and at least 10 % below the zero-drift error.
 """
 import sys
 
@@ -22,6 +24,7 @@ import pytest
 from cil_cuda_amd import abi, configs
 from cil_cuda_amd.colony import HipMD
 from oracle import mlp_oracle as M
+from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -163,9 +166,11 @@ def _near_tie(d):
 
 
 def test_knn_features_match_oracle(gpu):
-    md, p, box, col = small_ctx(3000, seed=7)
+    p, box, col = configs.relaxed_bulk_3d(3000, seed=7, precision=abi.PREC_FP32)
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
     n = col.total_cells()
-    F = md.knn_features(n).astype(np.float64)
+    F = md.knn_features(n).astype(np.float64)  # at step 0, forces computed by the call
     x = col.flat("pos").reshape(-1, 3)
     sig = col.flat("sigma").reshape(-1)
     spec = np.concatenate([np.full(sp["ncells"], s) for s, sp in enumerate(col.species)])
@@ -181,13 +186,29 @@ def test_knn_features_match_oracle(gpu):
         if _near_tie(r):
             continue
         checked += 1
-        want = bf16(ref[i])
+        want = bf16(ref[i, :66])
         tol = 2.0 ** -7 * np.abs(want) + 1e-5
-        if not np.all(np.abs(F[i] - want) <= tol):
-            bad = np.nonzero(np.abs(F[i] - want) > tol)[0]
+        if not np.all(np.abs(F[i, :66] - want) <= tol):
+            bad = np.nonzero(np.abs(F[i, :66] - want) > tol)[0]
             raise AssertionError(f"row {i}: features {bad[:8]} gpu {F[i][bad[:8]]} oracle {want[bad[:8]]}")
     say(f"knn: {checked}/{n} rows compared")
     assert checked >= 0.95 * n
+    # features 66-71 of every row: one-pass drift and noise sum of the macro-step from step 0
+    o = Oracle(p, box)
+    o.forces(col)
+    o.close()
+    temp = np.concatenate([p.dt / (p.species[s].zeta[0] * np.pi * (sp["sigma"][:sp["ncells"], 0] / 2) ** 2)
+                           for s, sp in enumerate(col.species)])
+    kT = np.concatenate([np.full(sp["ncells"], p.species[s].kT) for s, sp in enumerate(col.species)])
+    keys = [((s << 28) | c) << 1 for s, sp in enumerate(col.species) for c in range(sp["ncells"])]
+    z = M.fdt_noise_sum(p.seed, keys, np.sqrt(2 * kT * temp), 0)
+    want = M.macro_inputs(col.flat("frc").reshape(-1, 3), temp, z)
+    tol = 2.0 ** -7 * np.abs(want) + 1e-6
+    tol[:, :3] += 1e-3 * np.sqrt(np.mean(want[:, :3] ** 2))
+    err = np.abs(F[:, 66:] - want)
+    say(f"drift and noise features: worst error / bar {np.max(err / tol):.3f}; rms drift {np.sqrt(np.mean(want[:, :3] ** 2)):.3e}, "
+        f"rms noise {np.sqrt(np.mean(want[:, 3:] ** 2)):.3e} (x {M.BASE_SCALE})")
+    assert np.all(err <= tol)
     md.close()
 
 
@@ -226,11 +247,28 @@ def test_training_reduces_loss(gpu, trained):
     assert losses[-20:].mean() < 0.2 * losses[:5].mean()
 
 
-def test_trained_network_beats_zero_drift(gpu, trained):
+def test_zero_correction_is_one_euler_step(gpu):
+    """With a network that outputs 0 the macro-step is one Euler step of 8 dt (the one-pass drift
+    is kept in FP32: differences at FP32 rounding of the drift)."""
+    p, box, col = configs.relaxed_bulk_3d(2048, seed=5, precision=abi.PREC_FP32, kT=0.005)
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    md.forces(want=False)
+    md.mlp_set_params(np.zeros(abi.MLP_NPARAM, dtype=np.float32))
+    ts, rms, mx = md.surrogate_eval(0)
+    say(f"zero correction: rms network {rms[0]:.6e}, one Euler step {rms[2]:.6e}, zero drift {rms[1]:.6e}")
+    assert ts == 8
+    assert abs(rms[0] - rms[2]) <= 1e-5 * rms[2]
+    assert rms[2] < rms[1]
+    md.close()
+
+
+def test_trained_network_beats_one_euler_step(gpu, trained):
     md, ts, _ = trained
     ts, rms, mx = md.surrogate_eval(ts)
     say(f"8-step displacement error vs plain (rms / max, sigma): network {rms[0]:.3e} / {mx[0]:.3e}, "
         f"zero drift {rms[1]:.3e} / {mx[1]:.3e}, one Euler step of 8 dt {rms[2]:.3e} / {mx[2]:.3e}")
+    assert rms[0] < rms[2]
     assert rms[0] < 0.9 * rms[1]
 
 
