@@ -130,6 +130,7 @@ class SurrogateStats(C.Structure):
     _fields_ = [
         ("macro_steps", C.c_long), ("plain_steps", C.c_long), ("judged", C.c_long), ("failed", C.c_long),
         ("last_err_rms", C.c_double), ("last_err_max", C.c_double), ("worst_err_rms", C.c_double),
+        ("knn_list_builds", C.c_long), ("knn_fallbacks", C.c_long),
     ]
 
 

@@ -1516,6 +1516,7 @@ __device__ inline void k_pair_list_body(const StepParams* __restrict__ Pp, const
   // a re-sort whose grid barrier timed out left the sorted arrays half built: no list is built
   // from them (the host reports the error at its next check)
   if (rebuild && (__hip_atomic_load(S.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_SYNC)) return;
+  if (rebuild && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(S.rebuild + 2, 1u);  // list epoch
   int tp, k;
   if (!wave_type_slot(P, S, &tp, &k)) return;
   if (rebuild) build_list_of<D>(P, S, tp, k, bwin[threadIdx.x / WAVE]);
@@ -1688,6 +1689,45 @@ __device__ inline void cell_total_force(const StepParams& P, const Dev& S, int s
 // ======================================================================================
 // Fused update (updateVelocityPositions, CIL.cu:902-939), one thread per cell slot.
 // ======================================================================================
+
+// Neighbour-list bookkeeping of particle p moved to xn (list mode): the skin test against the
+// build position raises `next`, the flag of the force pass that follows, when it moved more
+// than skin / 2; otherwise the bin-local FP32 coordinates of its sorted slot (bins of the
+// last sort) are refreshed.  A set flag means that pass re-sorts anyway (and, right after an
+// upload, that no sorted slot exists yet), so the refresh is skipped then.  Every lane of the
+// wave that holds a particle calls it (one atomic per wave).
+template <int D>
+__device__ inline void list_track(const StepParams& P, const Dev& S, size_t p, const double* xn, double sig,
+                                  unsigned* next) {
+  bool finite = true;
+  float dr2 = 0.0f;
+  for (int d = 0; d < D; ++d) {
+    finite = finite && isfinite(xn[d]);
+    float dd = (float)(xn[d] - (double)S.xref[(size_t)d * S.P + p]);
+    if (P.box.pbc[d]) dd -= (float)P.box.L[d] * rintf(dd * (float)P.box.invL[d]);
+    dr2 = fmaf(dd, dd, dr2);
+  }
+  if (!finite) atomicOr(S.err, (unsigned)ERR_NONFINITE);
+  const bool moved = !(dr2 <= 0.25f * P.skin * P.skin);
+  // one atomic per wave (its first moved lane), none once the flag is up
+  const unsigned long long mv = __ballot(moved);
+  if (mv != 0ull && (int)(threadIdx.x & (WAVE - 1)) == __ffsll(mv) - 1 && *next == 0u) atomicOr(next, 1u);
+  if (!moved && *next == 0u) {
+    const int k = S.inv[p];
+    int bc[3];
+    unpack_bin(P, S.sbin[k], bc);
+    // the whole float4 slot in one aligned store (sigma rewritten with the value the sort
+    // stored), not a partial 12-byte write
+    float l4[4] = {0.f, 0.f, 0.f, (float)sig};
+    for (int d = 0; d < D; ++d) {
+      double l = xn[d] - bc[d] * P.w[d];
+      if (P.box.pbc[d]) l -= P.box.L[d] * rint(l * P.box.invL[d]);
+      l4[d] = (float)l;
+    }
+    ((float4*)S.spos)[k] = make_float4(l4[0], l4[1], l4[2], l4[3]);
+  }
+}
+
 template <typename T, int DIM, int NE>
 __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const Dev& S, int materialized, int emit_keys, int bid) {
   const StepParams& P = *Pp;
@@ -1779,38 +1819,7 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
       S.key[p] = b;
       S.rank[p] = atomicAdd(&S.count[b], 1);
     } else if (emit_keys == 2) {
-      // neighbour-list mode: skin test against the build positions, then the bin-local FP32
-      // coordinates of the particle's sorted slot (bins of the last sort).  A set rebuild flag
-      // means this force pass re-sorts anyway (and, right after an upload, that no sorted slot
-      // exists yet), so the refresh is skipped then.
-      bool finite = true;
-      float dr2 = 0.0f;
-      for (int d = 0; d < D; ++d) {
-        finite = finite && isfinite(xn[d]);
-        float dd = (float)(xn[d] - (double)S.xref[(size_t)d * S.P + p]);
-        if (P.box.pbc[d]) dd -= (float)P.box.L[d] * rintf(dd * (float)P.box.invL[d]);
-        dr2 = fmaf(dd, dd, dr2);
-      }
-      if (!finite) atomicOr(S.err, (unsigned)ERR_NONFINITE);
-      unsigned* next = next_pass_flag(S, tsu);  // flag of the force pass that follows
-      const bool moved = !(dr2 <= 0.25f * P.skin * P.skin);
-      // one atomic per wave (its first moved lane), none once the flag is up
-      const unsigned long long mv = __ballot(moved);
-      if (mv != 0ull && (int)(threadIdx.x & (WAVE - 1)) == __ffsll(mv) - 1 && *next == 0u) atomicOr(next, 1u);
-      if (!moved && *next == 0u) {
-        const int k = S.inv[p];
-        int bc[3];
-        unpack_bin(P, S.sbin[k], bc);
-        // the whole float4 slot in one aligned store (sigma rewritten with the value the sort
-        // stored), not a partial 12-byte write
-        float l4[4] = {0.f, 0.f, 0.f, (float)sig[e]};
-        for (int d = 0; d < D; ++d) {
-          double l = xn[d] - bc[d] * P.w[d];
-          if (P.box.pbc[d]) l -= P.box.L[d] * rint(l * P.box.invL[d]);
-          l4[d] = (float)l;
-        }
-        ((float4*)S.spos)[k] = make_float4(l4[0], l4[1], l4[2], l4[3]);
-      }
+      list_track<D>(P, S, p, xn, sig[e], next_pass_flag(S, tsu));
     }
   }
 }

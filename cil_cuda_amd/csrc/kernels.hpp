@@ -65,10 +65,10 @@ struct Dev {
   int* dblock;
   int* nnew;        // [CIL_MAX_SPECIES]
   // neighbour-list mode
-  unsigned* rebuild;  // [2] by the parity of a force pass's step index: nonzero = that pass
+  unsigned* rebuild;  // [0..1] by the parity of a force pass's step index: nonzero = that pass
                       // re-sorts and rebuilds the list (set by the skin test, division and
                       // host-initiated sorts; the update of step u clears word u & 1, whose
-                      // pass has run)
+                      // pass has run); [2] list epoch, counts the builds
   int* nl;            // [list_max / 4][P][4] groups of (sorted index << 10 | bin offset code);
                       // each source type starts on a group boundary
   int* nlt;           // [ntypes][P] entries per source type

@@ -470,6 +470,8 @@ typedef struct cil_surrogate_stats {
   double last_err_rms;  /* RMS over particles of |x_network - x_plain| at the last judgement */
   double last_err_max;
   double worst_err_rms;
+  long knn_list_builds; /* neighbour-list mode: kNN candidate-list builds */
+  long knn_fallbacks;   /* neighbour-list mode: particles whose kNN came from the bin walk */
 } cil_surrogate_stats;
 
 /* Number of network parameters (W1[128][72], b1, W2[128][128], b2, W3[3][128], b3; row-major,

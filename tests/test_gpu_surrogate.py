@@ -9,7 +9,10 @@ weights and inputs):
     (first Adam step with bias correction), to 1e-3 * lr, against the gradient's sign for
     >= 99.9 % of them;
   - kNN features 0-65: equal to the oracle's rows rounded to BF16 (one BF16 ulp) for every
-    particle without a near-tie (relative gap < 1e-4) among its 17 nearest distances;
+    particle without a near-tie (relative gap < 1e-4) among its 17 nearest distances, in
+    stencil mode and in neighbour-list mode after macro-steps and plain steps have moved the
+    particles away from the list's build (candidate lists, and with a skin that overfills them,
+    the fallback search over bins the particles have left);
   - features 66-71 (one-pass drift from the oracle's FP64 forces, noise sum from the oracle's
     draws): one BF16 ulp, plus 1e-3 of the drift's RMS for the FP32 forces;
   - training on plain trajectories (kT = 0.005): the trained network's 8-step displacement
@@ -165,12 +168,9 @@ def _near_tie(d):
     return np.any(gaps < 1e-4)
 
 
-def test_knn_features_match_oracle(gpu):
-    p, box, col = configs.relaxed_bulk_3d(3000, seed=7, precision=abi.PREC_FP32)
-    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
-    md.upload(col)
+def _check_knn_rows(F, col, box):
+    """Features 0-65 of every row without a near-tie against the oracle's kNN rows."""
     n = col.total_cells()
-    F = md.knn_features(n).astype(np.float64)  # at step 0, forces computed by the call
     x = col.flat("pos").reshape(-1, 3)
     sig = col.flat("sigma").reshape(-1)
     spec = np.concatenate([np.full(sp["ncells"], s) for s, sp in enumerate(col.species)])
@@ -193,6 +193,15 @@ def test_knn_features_match_oracle(gpu):
             raise AssertionError(f"row {i}: features {bad[:8]} gpu {F[i][bad[:8]]} oracle {want[bad[:8]]}")
     say(f"knn: {checked}/{n} rows compared")
     assert checked >= 0.95 * n
+
+
+def test_knn_features_match_oracle(gpu):
+    p, box, col = configs.relaxed_bulk_3d(3000, seed=7, precision=abi.PREC_FP32)
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    n = col.total_cells()
+    F = md.knn_features(n).astype(np.float64)  # at step 0, forces computed by the call
+    _check_knn_rows(F, col, box)
     # features 66-71 of every row: one-pass drift and noise sum of the macro-step from step 0
     o = Oracle(p, box)
     o.forces(col)
@@ -210,6 +219,66 @@ def test_knn_features_match_oracle(gpu):
         f"rms noise {np.sqrt(np.mean(want[:, 3:] ** 2)):.3e} (x {M.BASE_SCALE})")
     assert np.all(err <= tol)
     md.close()
+
+
+@pytest.mark.parametrize("skin", [0.3, 1.0])
+def test_knn_features_list_mode_after_moves(gpu, skin):
+    """Neighbour-list mode: features at the list's build, then after 3 macro-steps and 5 plain
+    steps (positions off the build by up to skin / 2, candidate lists reused).  Skin 1.0 makes
+    every candidate list overfull (about 290 particles within rb), so every row comes from the
+    fallback bin walk with its staleness margin."""
+    p, box, col = configs.relaxed_bulk_3d(3000, seed=8, precision=abi.PREC_FP32)
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    md.forces(want=False)
+    md.set_neighbour_list(True, skin, 512)
+    n = col.total_cells()
+    _check_knn_rows(md.knn_features(n).astype(np.float64), col, box)
+    md.mlp_set_params(np.zeros(abi.MLP_NPARAM, dtype=np.float32))
+    ts, st = md.surrogate_step(3, 0, judge_every=0, tol=1.0)
+    ts = md.step(5, ts)
+    F = md.knn_features(n).astype(np.float64)
+    c = col.copy()
+    md.download(c)
+    L = np.array(box.length[:3])
+    d = c.flat("pos").reshape(-1, 3) - col.flat("pos").reshape(-1, 3)
+    moved = np.abs(d - L * np.round(d / L)).max()
+    say(f"list mode skin {skin}: largest move {moved:.3e} sigma after {ts} steps")
+    assert moved > 1e-2
+    _check_knn_rows(F, c, box)
+    md.close()
+
+
+def test_list_mode_macro_steps_match_stencil_mode(gpu):
+    """The same sequence of macro-steps, judged macro-steps (plain result kept) and plain steps
+    with a zero network, in list mode (macro-steps keep the list's bookkeeping, the list is
+    rebuilt by its own skin test) and in stencil mode (re-sorted every macro-step): the two
+    differ only by the FP32 force summation order."""
+    p, box, col = configs.relaxed_bulk_3d(4096, seed=6, precision=abi.PREC_FP32)
+    ends = []
+    for list_on in (False, True):
+        md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+        md.upload(col)
+        md.forces(want=False)
+        if list_on:
+            md.set_neighbour_list(True, 0.3, 256)
+        md.mlp_set_params(np.zeros(abi.MLP_NPARAM, dtype=np.float32))
+        ts, st = md.surrogate_step(12, 0, judge_every=3, tol=1e9)
+        assert st["macro_steps"] == 8 and st["judged"] == 4
+        ts = md.step(9, ts)
+        ts, st = md.surrogate_step(5, ts, judge_every=0, tol=1.0)
+        assert ts == 8 * 17 + 9
+        c = col.copy()
+        md.download(c)
+        md.close()
+        ends.append(c.flat("pos").reshape(-1, 3))
+    L = np.array(box.length[:3])
+    d = ends[1] - ends[0]
+    d -= L * np.round(d / L)
+    moved = ends[0] - col.flat("pos").reshape(-1, 3)
+    moved -= L * np.round(moved / L)
+    say(f"list vs stencil mode after 145 steps: max |dx| {np.abs(d).max():.3e}, rms move {np.sqrt((moved ** 2).mean()):.3e}")
+    assert np.abs(d).max() < 1e-5
 
 
 def test_surrogate_requires_supported_setup(gpu):
@@ -247,13 +316,16 @@ def test_training_reduces_loss(gpu, trained):
     assert losses[-20:].mean() < 0.2 * losses[:5].mean()
 
 
-def test_zero_correction_is_one_euler_step(gpu):
+@pytest.mark.parametrize("list_on", [False, True])
+def test_zero_correction_is_one_euler_step(gpu, list_on):
     """With a network that outputs 0 the macro-step is one Euler step of 8 dt (the one-pass drift
     is kept in FP32: differences at FP32 rounding of the drift)."""
     p, box, col = configs.relaxed_bulk_3d(2048, seed=5, precision=abi.PREC_FP32, kT=0.005)
     md = HipMD(p, box, [sp["capacity"] for sp in col.species])
     md.upload(col)
     md.forces(want=False)
+    if list_on:
+        md.set_neighbour_list(True, 0.3, 256)
     md.mlp_set_params(np.zeros(abi.MLP_NPARAM, dtype=np.float32))
     ts, rms, mx = md.surrogate_eval(0)
     say(f"zero correction: rms network {rms[0]:.6e}, one Euler step {rms[2]:.6e}, zero drift {rms[1]:.6e}")
