@@ -236,6 +236,11 @@ def test_knn_features_list_mode_after_moves(gpu, skin):
     _check_knn_rows(md.knn_features(n).astype(np.float64), col, box)
     md.mlp_set_params(np.zeros(abi.MLP_NPARAM, dtype=np.float32))
     ts, st = md.surrogate_step(3, 0, judge_every=0, tol=1.0)
+    say(f"list mode skin {skin}: {st['knn_list_builds']} candidate-list builds, {st['knn_fallbacks']} fallback rows")
+    if skin == 1.0:
+        assert st["knn_fallbacks"] >= 3 * n // 2  # (nearly) every row of every macro-step
+    else:
+        assert st["knn_list_builds"] >= 1
     ts = md.step(5, ts)
     F = md.knn_features(n).astype(np.float64)
     c = col.copy()
