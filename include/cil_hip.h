@@ -460,7 +460,9 @@ const char* cil_hip_batch_last_error(cil_batch* batch);
  * division, walls, motility, pressure friction, drag or active noise; other set-ups return
  * CIL_E_INVALID.  The kNN search visits shells of bins around each particle until the 16th
  * neighbour is covered, up to 3 shells (fewer in boxes of < 7 bins per dimension); a 16th
- * neighbour beyond that returns CIL_E_INVALID.
+ * neighbour beyond that returns CIL_E_INVALID.  In neighbour-list mode the 16 nearest come from
+ * per-particle candidate lists rebuilt with the neighbour list (exact; particles whose list
+ * cannot guarantee it take the bin walk), and macro-steps keep the list's skin bookkeeping.
  * ------------------------------------------------------------------------------------- */
 typedef struct cil_surrogate_stats {
   long macro_steps;     /* macro-steps advanced by the network */
