@@ -254,11 +254,13 @@ def test_knn_features_list_mode_after_moves(gpu, skin):
     md.close()
 
 
-def test_list_mode_macro_steps_match_stencil_mode(gpu):
+@pytest.mark.parametrize("skin", [0.3, 0.1])
+def test_list_mode_macro_steps_match_stencil_mode(gpu, skin):
     """The same sequence of macro-steps, judged macro-steps (plain result kept) and plain steps
     with a zero network, in list mode (macro-steps keep the list's bookkeeping, the list is
     rebuilt by its own skin test) and in stencil mode (re-sorted every macro-step): the two
-    differ only by the FP32 force summation order."""
+    differ only by the FP32 force summation order.  With skin 0.1 (rebuild past 0.05 sigma) the
+    skin test of the macro-steps themselves triggers most rebuilds."""
     p, box, col = configs.relaxed_bulk_3d(4096, seed=6, precision=abi.PREC_FP32)
     ends = []
     for list_on in (False, True):
@@ -266,13 +268,17 @@ def test_list_mode_macro_steps_match_stencil_mode(gpu):
         md.upload(col)
         md.forces(want=False)
         if list_on:
-            md.set_neighbour_list(True, 0.3, 256)
+            md.set_neighbour_list(True, skin, 256)
         md.mlp_set_params(np.zeros(abi.MLP_NPARAM, dtype=np.float32))
         ts, st = md.surrogate_step(12, 0, judge_every=3, tol=1e9)
         assert st["macro_steps"] == 8 and st["judged"] == 4
         ts = md.step(9, ts)
         ts, st = md.surrogate_step(5, ts, judge_every=0, tol=1.0)
         assert ts == 8 * 17 + 9
+        if list_on:
+            say(f"list mode skin {skin}: {st['knn_list_builds']} candidate-list builds in 17 macro-steps")
+            if skin == 0.1:
+                assert st["knn_list_builds"] >= 5
         c = col.copy()
         md.download(c)
         md.close()
@@ -282,7 +288,8 @@ def test_list_mode_macro_steps_match_stencil_mode(gpu):
     d -= L * np.round(d / L)
     moved = ends[0] - col.flat("pos").reshape(-1, 3)
     moved -= L * np.round(moved / L)
-    say(f"list vs stencil mode after 145 steps: max |dx| {np.abs(d).max():.3e}, rms move {np.sqrt((moved ** 2).mean()):.3e}")
+    say(f"list (skin {skin}) vs stencil mode after 145 steps: max |dx| {np.abs(d).max():.3e}, "
+        f"rms move {np.sqrt((moved ** 2).mean()):.3e}")
     assert np.abs(d).max() < 1e-5
 
 
