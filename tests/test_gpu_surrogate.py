@@ -221,12 +221,12 @@ def test_knn_features_match_oracle(gpu):
     md.close()
 
 
-@pytest.mark.parametrize("skin", [0.3, 1.0])
+@pytest.mark.parametrize("skin", [0.3, 1.2])
 def test_knn_features_list_mode_after_moves(gpu, skin):
     """Neighbour-list mode: features at the list's build, then after 3 macro-steps and 5 plain
-    steps (positions off the build by up to skin / 2, candidate lists reused).  Skin 1.0 makes
-    every candidate list overfull (about 290 particles within rb), so every row comes from the
-    fallback bin walk with its staleness margin."""
+    steps (positions off the build by up to skin / 2, candidate lists reused).  Skin 1.2 makes
+    most candidate lists overfull (about 150 particles within rb = 3.2 sigma), so most rows come
+    from the fallback bin walk with its staleness margin."""
     p, box, col = configs.relaxed_bulk_3d(3000, seed=8, precision=abi.PREC_FP32)
     md = HipMD(p, box, [sp["capacity"] for sp in col.species])
     md.upload(col)
@@ -237,8 +237,8 @@ def test_knn_features_list_mode_after_moves(gpu, skin):
     md.mlp_set_params(np.zeros(abi.MLP_NPARAM, dtype=np.float32))
     ts, st = md.surrogate_step(3, 0, judge_every=0, tol=1.0)
     say(f"list mode skin {skin}: {st['knn_list_builds']} candidate-list builds, {st['knn_fallbacks']} fallback rows")
-    if skin == 1.0:
-        assert st["knn_fallbacks"] >= 3 * n // 2  # (nearly) every row of every macro-step
+    if skin > 1.0:
+        assert st["knn_fallbacks"] >= 2 * n  # most rows of the calls so far
     else:
         assert st["knn_list_builds"] >= 1
     ts = md.step(5, ts)
