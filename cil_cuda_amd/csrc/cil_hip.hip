@@ -1230,6 +1230,7 @@ __device__ inline void build_list_of(const StepParams& P, const Dev& S, int tp, 
   int bc[3];
   unpack_bin(P, S.sbin[k], bc);
   const float4 me = spos[k];
+  if (S.spos0) reinterpret_cast<float4*>(S.spos0)[k] = me;
   const float wx = (float)P.w[0], wy = (float)P.w[1], wz = (float)P.w[2];
   const bool mix = P.reach[0] == 0 && P.box.pbc[0], miy = P.reach[1] == 0 && P.box.pbc[1];
   const bool miz = D == 3 && P.reach[2] == 0 && P.box.pbc[2];

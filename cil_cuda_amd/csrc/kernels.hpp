@@ -73,6 +73,8 @@ struct Dev {
                       // each source type starts on a group boundary
   int* nlt;           // [ntypes][P] entries per source type
   float* xref;        // [4][P] positions (x, y, z) and diameter at the last build (particle order)
+  float* spos0;       // [P][4] or null: the sorted bin-local positions and diameter at the last
+                      // build (sorted order), kept for the surrogate's kNN candidate lists
   // integrator: 0 Euler (one force pass per step, at step index f); 1 stochastic Heun (two
   // passes per step: the corrector's at the step's own index with phase 1, the final one at the
   // next index with phase 0).  `phase` is set per launch.
