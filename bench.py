@@ -92,6 +92,9 @@ def surrogate_leg(md, ts, n, plain_value, args, p, box, col):
     """Train the drift surrogate on the GPU from plain trajectories of the same state, then
     time the surrogate-assisted step (judge every 16th macro-step, fallback above tol)."""
     floor, drift = surrogate.noise_floor(p, box, col)
+    order_floor = surrogate.noise_order_floor(p, box, col)
+    log(f"surrogate: drift's dependence on the order of the 8 increments (floor of any predictor given the "
+        f"start state and the noise sum) {order_floor:.3e}")
     log(f"surrogate: deterministic 8-step displacement rms {drift:.3e}; spread over noise (floor of any "
         f"start-state predictor) {floor:.3e}")
     t0 = time.perf_counter()
@@ -132,7 +135,7 @@ def surrogate_leg(md, ts, n, plain_value, args, p, box, col):
         "macro_steps": nmac, "judge_every": args.judge_every, "tol_sigma": args.tol, "stats": st,
         "eval_err_rms": {"network": rms[0], "zero_drift": rms[1], "euler_8dt": rms[2]},
         "eval_err_max": {"network": mx[0], "zero_drift": mx[1], "euler_8dt": mx[2]},
-        "noise_floor_rms": floor, "drift_rms": drift,
+        "noise_floor_rms": floor, "noise_order_floor_rms": order_floor, "drift_rms": drift,
         "training": {"samples": nsamp, "adam_steps": args.train_steps, "batch": 8192, "seconds": t_train,
                      "loss_first": float(losses[0]), "loss_last50": float(losses[-50:].mean())},
         "kernels_us": {k: kus[k] for k in ("sort", "knn", "mlp", "macro") if k in kus},

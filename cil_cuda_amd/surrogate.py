@@ -57,3 +57,37 @@ def noise_floor(params: abi.Params, box: abi.Box, colony, ts: int = 0, seeds=(10
     floor = float(np.sqrt(np.mean((d * d).sum(1)) / 2.0))
     drift = float(np.sqrt(np.mean(0.5 * ((T[0] ** 2).sum(1) + (T[1] ** 2).sum(1)))))
     return floor, drift
+
+
+def noise_order_floor(params: abi.Params, box: abi.Box, colony, ts: int = 0):
+    """RMS (per particle, length units) of the part of the 8-step drift that the macro-step's
+    inputs do not determine: the order of the 8 FDT increments.
+
+    8 plain steps run from the same start state twice, with the same increments in step order
+    and in reverse order (step indices ts+7 .. ts: the counter draws are keyed by the index).
+    The noise sum z and the one-pass drift b are the same for both, so half the RMS difference
+    of the end positions measures how the drift depends on the order.  For a linear force field
+    F = F0 + K x it is exactly the spread of the drift given (x0, z): the drift's noise term is
+    dt mu K sum_k (7 - k) eta_k, whose part beyond its mean given z, dt mu K sum_k (3.5 - k) eta_k,
+    changes sign under the reversal.  No predictor of (x0, b, z) can be more accurate."""
+    ends = []
+    for order in (1, -1):
+        md = HipMD(params, box, [sp["capacity"] for sp in colony.species])
+        try:
+            md.upload(colony)
+            md.forces(want=False)
+            if order == 1:
+                md.step(8, ts)
+            else:
+                for k in range(7, -1, -1):
+                    md.step(1, ts + k)
+            c = colony.copy()
+            md.download(c)
+            ends.append(c.flat("pos").reshape(-1, 3).astype(np.float64))
+        finally:
+            md.close()
+    L = np.array(box.length[:3], dtype=np.float64)
+    d = ends[0] - ends[1]
+    d -= L * np.round(d / L)
+    return float(np.sqrt(np.mean((d * d).sum(1))) / 2.0)
+

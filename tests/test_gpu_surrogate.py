@@ -367,6 +367,19 @@ def test_noise_floor_of_start_state_predictors(gpu):
     assert 0.0 < floor < drift
 
 
+def test_noise_order_floor_bounds_the_network(gpu, trained):
+    """The drift's dependence on the order of the 8 increments, which the macro-step's inputs do
+    not carry, is positive and below the trained network's error on the same kind of state."""
+    from cil_cuda_amd import surrogate
+
+    md, ts, _ = trained
+    p, box, col = configs.relaxed_bulk_3d(4096, seed=3, precision=abi.PREC_FP32, kT=0.005)
+    floor = surrogate.noise_order_floor(p, box, col)
+    ts, rms, mx = md.surrogate_eval(ts)
+    say(f"noise-order floor {floor:.3e} sigma; trained network {rms[0]:.3e} sigma")
+    assert 0.0 < floor < rms[0]
+
+
 def test_surrogate_step_judge_and_fallback(gpu, trained):
     md, ts, _ = trained
     md.surrogate_stats_reset()
