@@ -91,8 +91,11 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 def surrogate_leg(md, ts, n, plain_value, args, p, box, col):
     """Train the drift surrogate on the GPU from plain trajectories of the same state, then
     time the surrogate-assisted step (judge every 16th macro-step, fallback above tol)."""
-    floor, drift = surrogate.noise_floor(p, box, col)
-    order_floor = surrogate.noise_order_floor(p, box, col)
+    cur = md.download(col.copy())  # the state the network is trained and evaluated on
+    floor, drift = surrogate.noise_floor(p, box, cur, ts)
+    order_floor = surrogate.noise_order_floor(p, box, cur, ts)
+    base_err = surrogate.base_predictor_errors(p, box, cur, ts)
+    log(f"surrogate: network-free predictors on the same state: {', '.join(f'{k} {v:.3e}' for k, v in base_err.items())}")
     log(f"surrogate: drift's dependence on the order of the 8 increments (floor of any predictor given the "
         f"start state and the noise sum) {order_floor:.3e}")
     log(f"surrogate: deterministic 8-step displacement rms {drift:.3e}; spread over noise (floor of any "
@@ -136,6 +139,7 @@ def surrogate_leg(md, ts, n, plain_value, args, p, box, col):
         "eval_err_rms": {"network": rms[0], "zero_drift": rms[1], "euler_8dt": rms[2]},
         "eval_err_max": {"network": mx[0], "zero_drift": mx[1], "euler_8dt": mx[2]},
         "noise_floor_rms": floor, "noise_order_floor_rms": order_floor, "drift_rms": drift,
+        "base_predictors_rms": base_err,
         "training": {"samples": nsamp, "adam_steps": args.train_steps, "batch": 8192, "seconds": t_train,
                      "loss_first": float(losses[0]), "loss_last50": float(losses[-50:].mean())},
         "kernels_us": {k: kus[k] for k in ("sort", "knn", "mlp", "macro") if k in kus},

@@ -91,3 +91,64 @@ def noise_order_floor(params: abi.Params, box: abi.Box, colony, ts: int = 0):
     d -= L * np.round(d / L)
     return float(np.sqrt(np.mean((d * d).sum(1))) / 2.0)
 
+
+def base_predictor_errors(params: abi.Params, box: abi.Box, colony, ts: int = 0, alpha: float = 3.5 / 8):
+    """RMS errors (per particle, length units) against 8 plain steps from `colony` of three
+    network-free predictors of the 8-step displacement, each plus the exact noise sum z:
+      - zero drift: x0 + z;
+      - one Euler step of 8 dt (the surrogate's base b): x0 + b + z, b = 8 dt mu F(x0);
+      - midpoint: x0 + b_m + z, b_m = 8 dt mu F(x0 + alpha (b + z)), a second force pass.
+    For a linear force field the drift's mean given z is 8 dt mu F(x0 + (3.5 / 8) (b + z)) to first
+    order, so alpha = 3.5 / 8 leaves only the order term that noise_order_floor measures."""
+    import math
+
+    caps = [sp["capacity"] for sp in colony.species]
+    L = np.array(box.length[:3], dtype=np.float64)
+
+    def mi(d):
+        return d - L * np.round(d / L)
+
+    x0 = colony.flat("pos").reshape(-1, 3).astype(np.float64)
+    sig = colony.flat("sigma").reshape(-1)
+    spec = np.concatenate([np.full(sp["ncells"], s) for s, sp in enumerate(colony.species)])
+    zeta = np.array([params.species[s].zeta[0] for s in spec])
+    mudt = params.dt / (zeta * math.pi * (sig / 2) ** 2)
+
+    def forces(col):
+        md = HipMD(params, box, caps)
+        try:
+            md.upload(col)
+            return md.forces()[0]
+        finally:
+            md.close()
+
+    def run8(p, col):
+        md = HipMD(p, box, caps)
+        try:
+            md.upload(col)
+            md.forces(want=False)
+            md.step(8, ts)
+            c = col.copy()
+            md.download(c)
+            return c.flat("pos").reshape(-1, 3).astype(np.float64)
+        finally:
+            md.close()
+
+    b = 8 * mudt[:, None] * forces(colony)
+    free = abi.Params.from_buffer_copy(params)  # the same draws with every law off: z
+    for i in range(abi.MAX_TYPES):
+        for j in range(abi.MAX_TYPES):
+            free.law[i][j] = abi.LAW_NONE
+    z = mi(run8(free, colony) - x0)
+    x8 = run8(params, colony)
+    cm = colony.copy()
+    xm = x0 + alpha * (b + z)
+    for s, sp in enumerate(cm.species):
+        sp["pos"][: sp["ncells"], 0] = np.mod(xm[spec == s], L)
+    bm = 8 * mudt[:, None] * forces(cm)
+
+    def err(pred):
+        return float(np.sqrt(np.mean((mi(pred - x8) ** 2).sum(1))))
+
+    return {"zero_drift": err(x0 + z), "euler_8dt": err(x0 + b + z), "midpoint": err(x0 + bm + z)}
+

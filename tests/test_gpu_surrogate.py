@@ -380,6 +380,19 @@ def test_noise_order_floor_bounds_the_network(gpu, trained):
     assert 0.0 < floor < rms[0]
 
 
+def test_network_free_predictors(gpu):
+    """Of the predictors with the exact noise sum, a second force pass at x0 + (3.5/8)(b + z)
+    beats one Euler step of 8 dt, which beats zero drift, and none goes below the noise-order
+    floor."""
+    from cil_cuda_amd import surrogate
+
+    p, box, col = configs.relaxed_bulk_3d(4096, seed=3, precision=abi.PREC_FP32, kT=0.05)
+    e = surrogate.base_predictor_errors(p, box, col)
+    floor = surrogate.noise_order_floor(p, box, col)
+    say(f"network-free predictors (kT 0.05): {e}; noise-order floor {floor:.3e}")
+    assert floor < e["midpoint"] < e["euler_8dt"] < e["zero_drift"]
+
+
 def test_surrogate_step_judge_and_fallback(gpu, trained):
     md, ts, _ = trained
     md.surrogate_stats_reset()
