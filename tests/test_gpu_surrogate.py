@@ -293,6 +293,37 @@ def test_list_mode_macro_steps_match_stencil_mode(gpu, skin):
     assert np.abs(d).max() < 1e-5
 
 
+def test_long_list_mode_surrogate_run(gpu):
+    """100 macro-steps of a 16,384-particle bulk at kT 0.05 with a zero network (every
+    macro-step an Euler step of 8 dt; judged every 4th), in list mode and in stencil mode: the
+    list's own skin test drives its rebuilds, the candidate lists follow them, and the two runs
+    end within 1e-4 sigma of each other (FP32 summation order, amplified over 800 steps)."""
+    p, box, col = configs.relaxed_bulk_3d(16384, seed=11, precision=abi.PREC_FP32)
+    ends, stats = [], []
+    for list_on in (False, True):
+        md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+        md.upload(col)
+        md.forces(want=False)
+        if list_on:
+            md.set_neighbour_list(True, 0.3, 256)
+        md.mlp_set_params(np.zeros(abi.MLP_NPARAM, dtype=np.float32))
+        ts, st = md.surrogate_step(100, 0, judge_every=4, tol=1e9)
+        assert ts == 800
+        c = col.copy()
+        md.download(c)
+        md.close()
+        ends.append(c.flat("pos").reshape(-1, 3))
+        stats.append(st)
+    L = np.array(box.length[:3])
+    d = ends[1] - ends[0]
+    d -= L * np.round(d / L)
+    say(f"long run: list-mode stats {stats[1]}; list vs stencil max |dx| {np.abs(d).max():.3e}")
+    assert np.all(np.isfinite(ends[1]))
+    assert stats[1]["macro_steps"] == 75 and stats[1]["judged"] == 25
+    assert stats[1]["knn_list_builds"] >= 3
+    assert np.abs(d).max() < 1e-4
+
+
 def test_surrogate_requires_supported_setup(gpu):
     p, box, col = configs.plain_2d_sc(n_cells=64, L=16.0, seed=2)
     md = HipMD(p, box, [sp["capacity"] for sp in col.species])
