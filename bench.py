@@ -32,7 +32,7 @@ N_PARTICLES = 1 << 20
 # Algorithmic bytes per particle per launch (external inputs once + outputs once) of the
 # pair kernel in 3D FP32: read sorted x,y,z,sigma (16) + tag (4); write force (12) + pressure (4).
 PAIR_BYTES_PER_PARTICLE = 36
-# Whole FP32 3D step (update+keys 72, scan 4, scatter+binsort 60, pair 36)
+# Whole FP32 3D step: update+keys 72, scan 4, scatter+binsort 60, pair 36
 STEP_BYTES_PER_PARTICLE = 200
 
 
@@ -193,8 +193,8 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the multi-GPU "
-              "sharding is a design (see DESIGN.md), not a run", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (got --gpus {args.gpus}); sharding a colony across GPUs "
+              "is designed but not implemented", file=sys.stderr)
         sys.exit(2)
 
     t0 = time.perf_counter()
