@@ -176,7 +176,10 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="pin the pair kernel block size (with --bins-per-cutoff)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-list", action="store_true", help="stencil pair kernel instead of the neighbour list")
-    ap.add_argument("--skin", type=float, default=0.3)
+    ap.add_argument("--skin", type=float, default=0.6, help="neighbour-list skin (the outer list's, with --inner-skin)")
+    ap.add_argument("--inner-skin", type=float, default=0.2, help="two-level lists: the inner list's skin (0: one level)")
+    ap.add_argument("--surrogate-skin", type=float, default=0.3,
+                    help="skin of the one-level lists the surrogate legs use")
     ap.add_argument("--repeats", type=int, default=5,
                     help="steady-state repeats of at least one second each (median reported beside value)")
     ap.add_argument("--list-max", type=int, default=256)
@@ -206,8 +209,9 @@ def main():
     md.upload(col)
     md.forces(want=False)
     if not args.no_list:
-        md.set_neighbour_list(True, args.skin, args.list_max)
-        log(f"neighbour list on: skin {args.skin} sigma, {args.list_max} entries per particle")
+        md.set_neighbour_list(True, args.skin, args.list_max, inner_skin=args.inner_skin)
+        log(f"neighbour list on: skin {args.skin} sigma, inner skin {args.inner_skin} sigma, "
+            f"{args.list_max} entries per particle")
     tuned = None
     if not args.bins_per_cutoff:  # stencil: bins x block x kernel; list mode: bins per cut-off
         tuned = md.autotune(reps=3)
@@ -297,7 +301,8 @@ def main():
         "config": {"workload": "3D periodic bulk, 1048576 single discs, phi=0.6, lognormal sigma CV 0.1, "
                                "LJ 2.5 sigma (species 0 pairs) + soft core, dt=1e-4, kT=0.05",
                    "particles": n, "bins": nb, "bin_width": w, "autotuned": tuned,
-                   "neighbour_list": None if args.no_list else {"skin": args.skin, "max_neighbours": args.list_max}},
+                   "neighbour_list": None if args.no_list else {"skin": args.skin, "inner_skin": args.inner_skin,
+                                                                 "max_neighbours": args.list_max}},
         "kernels_us": fam,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
@@ -319,6 +324,8 @@ def main():
         "graph_path": {"steps": gsteps, "ms_per_step": graph_elapsed * 1e3 / gsteps, "value": n * gsteps / graph_elapsed},
     }
     if not args.no_surrogate:
+        if not args.no_list:  # the surrogate runs on one-level lists
+            md.set_neighbour_list(True, args.surrogate_skin, args.list_max)
         out["surrogate"] = surrogate_leg(md, ts, n, value, args, p, box, col)
     md.close()
     if not args.no_surrogate:
@@ -334,7 +341,7 @@ def main():
             m2.set_tuning(args.bins_per_cutoff, args.block)
         m2.upload(col)
         if not args.no_list:
-            m2.set_neighbour_list(True, args.skin, args.list_max)
+            m2.set_neighbour_list(True, args.surrogate_skin, args.list_max)
         m2.forces(want=False)
         res = surrogate_leg(m2, 0, n, value, args, q, box, col)
         res["kT"] = 0.005

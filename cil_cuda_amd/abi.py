@@ -172,6 +172,7 @@ def lib():
     L.cil_hip_set_timing.argtypes = [vp, C.c_int]
     L.cil_hip_set_kernel.argtypes = [vp, C.c_int]
     L.cil_hip_set_neighbour_list.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+    L.cil_hip_set_neighbour_list_inner.argtypes = [vp, C.c_double]
     L.cil_hip_autotune.argtypes = [vp, C.c_int, PI, PI, PI, PD]
     PF = C.POINTER(C.c_float)
     PL = C.POINTER(C.c_long)
@@ -239,7 +240,7 @@ EXPORTS = (
     "cil_hip_create", "cil_hip_destroy", "cil_hip_upload", "cil_hip_step", "cil_hip_download",
     "cil_hip_forces", "cil_hip_pairs", "cil_hip_pairs_noalloc", "cil_hip_check_positions", "cil_hip_set_external_noise",
     "cil_hip_num_cells", "cil_hip_timings", "cil_hip_bin_info", "cil_hip_set_tuning", "cil_hip_set_timing", "cil_hip_set_kernel", "cil_hip_autotune",
-    "cil_hip_set_neighbour_list",
+    "cil_hip_set_neighbour_list", "cil_hip_set_neighbour_list_inner",
     "cil_hip_sizeof", "cil_hip_last_error", "cil_hip_version",
     "cil_hip_config_read", "cil_hip_config_init", "cil_hip_dsfmt_fill", "cil_hip_gsd_create", "cil_hip_gsd_write_chunk",
     "cil_hip_gsd_end_frame", "cil_hip_gsd_close", "cil_hip_gsd_write_frame", "cil_hip_gsd_write_frame_ctx",

@@ -215,9 +215,13 @@ class HipMD:
                                                              C.byref(n), 1 if clear else 0), "division_log")
         return pos[:n.value], ts[:n.value]
 
-    def set_neighbour_list(self, on=True, skin=0.3, max_neighbours=256):
+    def set_neighbour_list(self, on=True, skin=0.3, max_neighbours=256, inner_skin=0.0):
+        """Neighbour-list mode; inner_skin > 0 adds the inner list of two-level lists."""
         _check(self.L, self.ctx, self.L.cil_hip_set_neighbour_list(self.ctx, 1 if on else 0, float(skin),
                                                                    int(max_neighbours)), "set_neighbour_list")
+        if on and inner_skin:
+            _check(self.L, self.ctx, self.L.cil_hip_set_neighbour_list_inner(self.ctx, float(inner_skin)),
+                   "set_neighbour_list_inner")
 
     FAMILIES = ("update", "division", "sort", "pair", "noise", "guard", "build", "knn", "mlp", "macro")
 

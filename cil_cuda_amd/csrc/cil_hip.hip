@@ -1432,13 +1432,49 @@ __device__ __forceinline__ int4 list_group(const char* __restrict__ nlb, unsigne
 // `off` of the list (groups `stride` bytes apart), in list order (the summation order is the
 // list order).  LC is the segment's law class, MI whether any dimension uses the single-bin
 // minimum image.
-template <int D, int LC, bool MI, bool SH>
+// The inner list a refresh pass writes for one particle (two-level lists): the entries of the
+// (outer) list within cut-off + inner skin, in list order, in groups of four; each source type
+// starts on a group boundary, as in the list build.
+struct InnerList {
+  int4* nl4;
+  size_t NP;
+  int k, ngroups;
+  float rca, rcb, skin;  // the list radius of the current source type: (rca (s_i + s_j) + rcb + skin) 1.001
+  int slot, m, b0, b1, b2, b3;
+  bool over;
+  __device__ __forceinline__ void append(int e) {
+    const int s = slot & 3;
+    b0 = s == 0 ? e : b0;
+    b1 = s == 1 ? e : b1;
+    b2 = s == 2 ? e : b2;
+    b3 = s == 3 ? e : b3;
+    ++slot;
+    ++m;
+    if (s == 3) {
+      const int g = (slot >> 2) - 1;
+      if (g < ngroups) nl4[(size_t)g * NP + k] = make_int4(b0, b1, b2, b3);
+      else over = true;
+    }
+  }
+  // close the current source type: its partial last group, and its entry count
+  __device__ __forceinline__ void end_type(int* count) {
+    if (slot & 3) {
+      const int g = slot >> 2;
+      if (g < ngroups) nl4[(size_t)g * NP + k] = make_int4(b0, b1, b2, b3);
+      else over = true;
+      slot = (slot + 3) & ~3;
+    }
+    *count = m;
+    m = 0;
+  }
+};
+
+template <int D, int LC, bool MI, bool SH, bool RF>
 __device__ __forceinline__ void list_segment(const StepParams& P, const Dev& S, const Law32& Lw, const float4* tab,
                                              const float4& me, int k, int pidx, uint32_t ts, float cadh, int n,
-                                             unsigned& off, unsigned stride, float& Fx, float& Fy, float& Fz,
-                                             float& pr, int& nguard) {
+                                             const char* __restrict__ nlb, unsigned& off, unsigned stride, float& Fx,
+                                             float& Fy, float& Fz, float& pr, int& nguard, InnerList& il) {
   const float4* __restrict__ spos = (const float4*)S.spos;
-  const char* __restrict__ nlb = (const char*)S.nl;
   auto term = [&](unsigned e, const float4& r, const float4& o) {
     float dx = r.x - (me.x - o.x);
     float dy = r.y - (me.y - o.y);
@@ -1449,6 +1485,10 @@ __device__ __forceinline__ void list_segment(const StepParams& P, const Dev& S, 
       if (D == 3 && P.reach[2] == 0 && P.box.pbc[2]) dz -= (float)P.box.L[2] * rintf(dz * (float)P.box.invL[2]);
     }
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    if (RF) {  // refresh pass: keep the entry in the inner list when within its radius
+      const float rl = (fmaf(il.rca, r.w + me.w, il.rcb) + il.skin) * 1.001f;
+      if (!(r2 > rl * rl)) il.append((int)e);
+    }
     float f0;
     if (LC == CIL_LAW_LJ) {
       bool guard;
@@ -1484,14 +1524,23 @@ __device__ __forceinline__ void list_segment(const StepParams& P, const Dev& S, 
     const int4 g = list_group(nlb, off);
     off += stride;
     const unsigned e[4] = {(unsigned)g.x, (unsigned)g.y, (unsigned)g.z, (unsigned)g.w};
-    float4 r[4];
+    if (RF) {  // refresh pass: two gathers in flight, leaving registers for the inner list's state
 #pragma unroll
-    for (int u = 0; u < 4; ++u) r[u] = spos[e[u] >> 10];
-    float4 o[4];
+      for (int h = 0; h < 4; h += 2) {
+        const float4 r0 = spos[e[h] >> 10], r1 = spos[e[h + 1] >> 10];
+        term(e[h], r0, tab[e[h] & 1023u]);
+        term(e[h + 1], r1, tab[e[h + 1] & 1023u]);
+      }
+    } else {
+      float4 r[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) o[u] = tab[e[u] & 1023u];
+      for (int u = 0; u < 4; ++u) r[u] = spos[e[u] >> 10];
+      float4 o[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) term(e[u], r[u], o[u]);
+      for (int u = 0; u < 4; ++u) o[u] = tab[e[u] & 1023u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) term(e[u], r[u], o[u]);
+    }
   }
   if (i < n) {  // last, partial group: 1 to 3 entries
     const int4 g = list_group(nlb, off);
@@ -1534,30 +1583,58 @@ __device__ inline void k_pair_list_body(const StepParams* __restrict__ Pp, const
   unsigned off = 16u * (unsigned)k;
   float Fx = 0.f, Fy = 0.f, Fz = 0.f, pr = 0.f;
   int nguard = 0;
-  for (int tq = 0; tq < P.ntypes; ++tq) {
-    const int n = S.nlt[(size_t)tq * NP + k];
-    if (n == 0) continue;
-    const Law32 Lw = law32(P, tp, tq);
-    const float cadh = (tq / nE == spec_p) ? 1.0f : ca_p;  // scale for f0 >= 0 (MD.cu:345-360)
-#define CIL_SEG(LC, MIV, SHV) \
-  list_segment<D, LC, MIV, SHV>(P, S, Lw, tab, me, k, pidx, ts, cadh, n, off, stride, Fx, Fy, Fz, pr, nguard)
-    if (Lw.law == CIL_LAW_LJ) {
-      if (Lw.shifted) {
-        if (mi) CIL_SEG(CIL_LAW_LJ, true, true); else CIL_SEG(CIL_LAW_LJ, false, true);
-      } else {
-        if (mi) CIL_SEG(CIL_LAW_LJ, true, false); else CIL_SEG(CIL_LAW_LJ, false, false);
-      }
-    } else if (Lw.law == CIL_LAW_SC) {
-      if (Lw.shifted) {
-        if (mi) CIL_SEG(CIL_LAW_SC, true, true); else CIL_SEG(CIL_LAW_SC, false, true);
-      } else {
-        if (mi) CIL_SEG(CIL_LAW_SC, true, false); else CIL_SEG(CIL_LAW_SC, false, false);
-      }
-    } else {
-      if (mi) CIL_SEG(LC_CONTACT, true, false); else CIL_SEG(LC_CONTACT, false, false);
-    }
+  // two-level lists: a refresh pass evaluates the outer list and writes the inner one; other
+  // passes evaluate the inner list.  The two loops are compiled apart, so that the inner list's
+  // append state is live only in the refresh pass.
+  const bool two = S.nl_in != nullptr;
+  const bool refresh = two && (rebuild || *refresh_flag(S, step_ts(S)) != 0u);
+  auto run_types = [&](auto rfc) {
+    constexpr bool RF = decltype(rfc)::value;
+    const bool inner = two && !RF;
+    const char* __restrict__ nlb = inner ? (const char*)S.nl_in : (const char*)S.nl;
+    const int* __restrict__ nlt = inner ? S.nlt_in : S.nlt;
+    InnerList il{(int4*)S.nl_in, NP, k, P.list_max >> 2, 0.f, 0.f, P.skin_in, 0, 0, 0, 0, 0, 0, false};
+    for (int tq = 0; tq < P.ntypes; ++tq) {
+      const int n = nlt[(size_t)tq * NP + k];
+      if (n > 0) {
+        const Law32 Lw = law32(P, tp, tq);
+        const float cadh = (tq / nE == spec_p) ? 1.0f : ca_p;  // scale for f0 >= 0 (MD.cu:345-360)
+        if (RF) {
+          il.rca = (float)P.rca_tt[tp][tq];
+          il.rcb = (float)P.rcb_tt[tp][tq];
+        }
+#define CIL_SEG(LC, MIV, SHV)                                                                                 \
+  list_segment<D, LC, MIV, SHV, RF>(P, S, Lw, tab, me, k, pidx, ts, cadh, n, nlb, off, stride, Fx, Fy, Fz, pr, \
+                                    nguard, il)
+        if (Lw.law == CIL_LAW_LJ) {
+          if (Lw.shifted) {
+            if (mi) CIL_SEG(CIL_LAW_LJ, true, true); else CIL_SEG(CIL_LAW_LJ, false, true);
+          } else {
+            if (mi) CIL_SEG(CIL_LAW_LJ, true, false); else CIL_SEG(CIL_LAW_LJ, false, false);
+          }
+        } else if (Lw.law == CIL_LAW_SC) {
+          if (Lw.shifted) {
+            if (mi) CIL_SEG(CIL_LAW_SC, true, true); else CIL_SEG(CIL_LAW_SC, false, true);
+          } else {
+            if (mi) CIL_SEG(CIL_LAW_SC, true, false); else CIL_SEG(CIL_LAW_SC, false, false);
+          }
+        } else {
+          if (mi) CIL_SEG(LC_CONTACT, true, false); else CIL_SEG(LC_CONTACT, false, false);
+        }
 #undef CIL_SEG
-  }
+      }
+      if (RF) il.end_type(S.nlt_in + (size_t)tq * NP + k);
+    }
+    if (RF) {
+      if (il.over) {  // capacity error: an empty inner list keeps later passes inside the allocation
+        for (int tq = 0; tq < P.ntypes; ++tq) S.nlt_in[(size_t)tq * NP + k] = 0;
+        atomicOr(S.err, (unsigned)ERR_LIST);
+      }
+      for (int d = 0; d < D; ++d) S.xref_in[(size_t)d * NP + pidx] = (float)S.pos[(size_t)d * NP + pidx];
+    }
+  };
+  if (refresh) run_types(std::true_type{});
+  else run_types(std::false_type{});
   if (nguard > 0) {
     float g[3] = {0.f, 0.f, 0.f};
     guard_force(P, S, k, nguard, g);
@@ -1713,6 +1790,17 @@ __device__ inline void list_track(const StepParams& P, const Dev& S, size_t p, c
   // one atomic per wave (its first moved lane), none once the flag is up
   const unsigned long long mv = __ballot(moved);
   if (mv != 0ull && (int)(threadIdx.x & (WAVE - 1)) == __ffsll(mv) - 1 && *next == 0u) atomicOr(next, 1u);
+  if (S.xref_in) {  // two-level lists: the inner list's own skin test, against its last refresh
+    float di2 = 0.0f;
+    for (int d = 0; d < D; ++d) {
+      float dd = (float)(xn[d] - (double)S.xref_in[(size_t)d * S.P + p]);
+      if (P.box.pbc[d]) dd -= (float)P.box.L[d] * rintf(dd * (float)P.box.invL[d]);
+      di2 = fmaf(dd, dd, di2);
+    }
+    const unsigned long long mi = __ballot(!(di2 <= 0.25f * P.skin_in * P.skin_in));
+    unsigned* next_in = next + 4;  // the refresh flag of the same pass
+    if (mi != 0ull && (int)(threadIdx.x & (WAVE - 1)) == __ffsll(mi) - 1 && *next_in == 0u) atomicOr(next_in, 1u);
+  }
   if (!moved && *next == 0u) {
     const int k = S.inv[p];
     int bc[3];
@@ -1734,7 +1822,10 @@ __device__ inline void k_update_body(const StepParams* __restrict__ Pp, const De
   const StepParams& P = *Pp;
   // list mode: the flag of the previous force pass (same step parity as this update) has been
   // used; the skin test below sets the other one, for this step's force pass
-  if (emit_keys == 2 && bid == 0 && threadIdx.x == 0) *rebuild_flag(S, step_ts(S)) = 0u;
+  if (emit_keys == 2 && bid == 0 && threadIdx.x == 0) {
+    *rebuild_flag(S, step_ts(S)) = 0u;
+    if (S.xref_in) *refresh_flag(S, step_ts(S)) = 0u;
+  }
   int cslot = bid * blockDim.x + threadIdx.x;
   if (cslot >= S.C) return;
   int s, c;

@@ -182,6 +182,7 @@ struct StepParams {
   int list_mode;
   int list_max;     // list capacity per particle
   float skin;
+  float skin_in;    // two-level lists: the inner list's skin (0: one level)
   float list_rca_max;  // largest rca of any type pair with a law: a pair cut-off grows by at most
                        // list_rca_max per unit of diameter growth of either particle
   cil_wall walls[CIL_MAX_WALLS];

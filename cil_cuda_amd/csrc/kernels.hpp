@@ -75,6 +75,13 @@ struct Dev {
   float* xref;        // [4][P] positions (x, y, z) and diameter at the last build (particle order)
   float* spos0;       // [P][4] or null: the sorted bin-local positions and diameter at the last
                       // build (sorted order), kept for the surrogate's kNN candidate lists
+  // two-level lists (null: one level): the inner list holds the (outer) list's entries within
+  // cut-off + inner skin at its last refresh, in the same order; a pass refreshes it, while
+  // evaluating the outer list, when the outer list was rebuilt or a particle has moved more
+  // than inner skin / 2 since the refresh (flags rebuild[4..5], by parity like [0..1])
+  int* nl_in;         // [list_max / 4][P][4], the layout of nl
+  int* nlt_in;        // [ntypes][P]
+  float* xref_in;     // [3][P] positions at the last refresh (particle order)
   // integrator: 0 Euler (one force pass per step, at step index f); 1 stochastic Heun (two
   // passes per step: the corrector's at the step's own index with phase 1, the final one at the
   // next index with phase 0).  `phase` is set per launch.
@@ -118,6 +125,8 @@ __device__ inline int step_ts(const Dev& S) { return *S.ts + S.ts_off; }
 // passes word 1, final passes word 0).  A launch's own pass, and the update's previous pass,
 // is the one of its (step index, phase).
 __device__ inline unsigned* rebuild_flag(const Dev& S, int f) { return S.rebuild + (S.heun ? S.phase : (f & 1)); }
+// Inner-list refresh flag of the force pass at step index f (two-level lists, Euler only).
+__device__ inline unsigned* refresh_flag(const Dev& S, int f) { return S.rebuild + 4 + (f & 1); }
 // Flag of the force pass that follows an update launched at step index f.
 __device__ inline unsigned* next_pass_flag(const Dev& S, int f) {
   return S.rebuild + (S.heun ? 1 - S.phase : ((f + 1) & 1));

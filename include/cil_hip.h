@@ -332,6 +332,18 @@ int cil_hip_set_timing(cil_ctx* ctx, int on);
  * cut-off + skin while it is on. */
 int cil_hip_set_neighbour_list(cil_ctx* ctx, int on, double skin, int max_neighbours);
 
+/* Two-level neighbour lists (project addition; call after cil_hip_set_neighbour_list, which
+ * returns to one level).  The list above (cut-off + skin) becomes the outer list; an inner list
+ * holds its entries within cut-off + skin_in (0 < skin_in < skin), in the same order, and the
+ * passes evaluate the inner list.  A pass refreshes the inner list from the outer one, while
+ * evaluating the outer list, when the outer list was rebuilt or a particle has moved more than
+ * skin_in / 2 since the last refresh.  The pairs within their cut-off, and the order in which
+ * they are summed, are those of a one-level list, so forces are bit-identical to it; a large
+ * skin then makes re-sorts and list builds rare while the per-step list stays short.  Needs the
+ * Euler integrator and no division (CIL_E_INVALID otherwise); skin_in = 0 returns to one level.
+ * The drift surrogate needs one-level lists. */
+int cil_hip_set_neighbour_list_inner(cil_ctx* ctx, double skin_in);
+
 /* ---------------------------------------------------------------------------------------
  * Reference run configurations (project addition: the reference's own JSON input files).
  *

@@ -160,6 +160,10 @@ class Session {
     run_assert(cil_hip_set_neighbour_list(ctx_, on ? 1 : 0, skin, max_neighbours), ctx_,
                "cil_hip_set_neighbour_list");
   }
+  // Two-level lists over the list above (project addition; cil_hip_set_neighbour_list_inner)
+  void setInnerList(double skin_in) {
+    run_assert(cil_hip_set_neighbour_list_inner(ctx_, skin_in), ctx_, "cil_hip_set_neighbour_list_inner");
+  }
   // Division records for the HDF5 /div_app group (project addition; cil_hip_set_division_log)
   void setDivisionLog(bool on) {
     run_assert(cil_hip_set_division_log(ctx_, on ? 1 : 0), ctx_, "cil_hip_set_division_log");

@@ -10,7 +10,8 @@ median S_p) against the FP64 oracle):
     the FP32 stencil kernel and an FP64 run agree within 1e-4 sigma (FP32 rounding, amplified
     by the discontinuous LJ cut-off), and the forces of the end state still meet the bar;
   - division and drain: growing channels give the same division events and cell counts as
-    the stencil kernel, positions within 1e-5 sigma.
+    the stencil kernel, positions within 1e-5 sigma;
+  - two-level lists: bit-identical to one-level lists of the same skin.
 """
 import sys
 
@@ -18,7 +19,7 @@ import numpy as np
 import pytest
 
 from cil_cuda_amd import abi, configs
-from cil_cuda_amd.colony import HipMD
+from cil_cuda_amd.colony import HipMD, _check
 from oracle import oracle as O
 
 sys.path.insert(0, __file__.rsplit("/", 1)[0])
@@ -213,6 +214,72 @@ def test_list_mode_rejects_fp64(gpu):
     with pytest.raises(abi.CilError):
         md.set_neighbour_list(True)
     md.close()
+
+
+def _run_lists(p, box, col, nsteps, skin, inner):
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    md.forces(want=False)
+    md.set_neighbour_list(True, skin, 256, inner_skin=inner)
+    ts = md.step(nsteps, 0)
+    frc, _ = md.forces()
+    out = col.copy()
+    md.download(out)
+    md.close()
+    assert ts == nsteps
+    return out.flat("pos").reshape(-1, 3), frc
+
+
+@pytest.mark.parametrize("inner", [0.02, 0.15, 0.45])
+def test_two_level_lists_bit_identical_to_one_level(gpu, inner):
+    """Two-level lists (outer skin 0.5) against one-level lists of the same skin: the inner list
+    holds the outer list's entries within cut-off + inner skin in the same order, so every step
+    sums the same pairs in the same order.  Over 1,500 noisy steps of a 32,768-particle bulk
+    (kT 0.05), with inner refreshes from almost every step (inner skin 0.02) to rarely (0.45)
+    and outer rebuilds in between, positions and the end forces are bit-identical."""
+    p, box, col = configs.relaxed_bulk_3d(32768, seed=5, precision=abi.PREC_FP32)
+    x1, f1 = _run_lists(p, box, col, 1500, 0.5, 0.0)
+    x2, f2 = _run_lists(p, box, col, 1500, 0.5, inner)
+    x0 = col.flat("pos").reshape(-1, 3)
+    L = np.array(box.length[:3])
+    d = x1 - x0
+    d -= L * np.round(d / L)
+    say(f"two-level lists, inner skin {inner}: largest move {np.abs(d).max():.3f} sigma; identical "
+        f"{np.array_equal(x1, x2)} {np.array_equal(f1, f2)}")
+    assert np.abs(d).max() > 0.3  # beyond half the outer skin: the outer list was rebuilt on the way
+    assert np.array_equal(x1, x2)
+    assert np.array_equal(f1, f2)
+
+
+def test_two_level_lists_reject_unsupported_setups(gpu):
+    p, box, col = configs.relaxed_bulk_3d(4096, seed=6, precision=abi.PREC_FP32)
+    md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+    md.upload(col)
+    md.forces(want=False)
+    with pytest.raises(abi.CilError):  # no list
+        _check_inner(md, 0.1)
+    md.set_neighbour_list(True, 0.3, 256)
+    with pytest.raises(abi.CilError):  # inner skin not below the skin
+        _check_inner(md, 0.3)
+    _check_inner(md, 0.1)
+    md.mlp_init(1)
+    with pytest.raises(abi.CilError):  # the surrogate needs one-level lists
+        md.surrogate_step(1, 0, judge_every=0, tol=1.0)
+    _check_inner(md, 0.0)  # back to one level
+    ts, _ = md.surrogate_step(1, 0, judge_every=0, tol=1.0)
+    assert ts == 8
+    md.close()
+    q, box2, col2 = _theta_growing()
+    m2 = HipMD(q, box2, [sp["capacity"] for sp in col2.species])
+    m2.upload(col2)
+    m2.set_neighbour_list(True, 0.3, 256)
+    with pytest.raises(abi.CilError):  # division
+        _check_inner(m2, 0.1)
+    m2.close()
+
+
+def _check_inner(md, skin_in):
+    _check(md.L, md.ctx, md.L.cil_hip_set_neighbour_list_inner(md.ctx, float(skin_in)), "inner")
 
 
 def _theta_growing(n_cells=120, L=24.0, seed=7):
