@@ -1656,8 +1656,10 @@ k_pair_list(const StepParams* __restrict__ Pp, Dev S) {
 }
 
 // Both rebuild flags (either step parity) to v.
-__global__ void k_set_flags(unsigned* flag, unsigned v) {
+// Rebuild flags of both parities = v, inner-list refresh flags of both parities = vin.
+__global__ void k_set_flags(unsigned* flag, unsigned v, unsigned vin) {
   if (threadIdx.x < 2 && blockIdx.x == 0) flag[threadIdx.x] = v;
+  else if (threadIdx.x < 4 && blockIdx.x == 0) flag[4 + (threadIdx.x - 2)] = vin;
 }
 
 

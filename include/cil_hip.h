@@ -308,7 +308,8 @@ int cil_hip_division_log(cil_ctx* ctx, int s, double* pos, int* ts, int cap, int
 /* Autotuner (project addition): times `reps` force passes for every bins-per-cut-off /
  * pair-block / kernel-variant candidate, keeps the fastest and reports it.  In neighbour-list
  * mode it chooses the bins per cut-off (1-3) that minimise a pass without a rebuild plus a
- * rebuild pass spread over 45 steps (block and variant are reported unchanged).  Needs forces
+ * rebuild pass spread over 45 x (skin / 0.3)^2 steps, plus with two-level lists an inner-list
+ * refresh spread over 45 x (skin_in / 0.3)^2 steps (block and variant are reported unchanged).  Needs forces
  * computed by cil_hip_forces or a step (the state is not advanced). */
 int cil_hip_autotune(cil_ctx* ctx, int reps, int* best_bins, int* best_block, int* best_variant, double* best_us);
 

@@ -383,14 +383,16 @@ def test_two_list_contexts_step_concurrently(gpu):
         assert np.array_equal(seq[k], par[k]), k
 
 
-def test_list_autotune_keeps_the_state(gpu):
-    """The list-mode autotuner times rebuild and plain passes at 1-3 bins per cut-off at the
-    current positions: the state is not advanced, the forces it leaves are those of a fresh
-    context with the chosen bins (bit for bit), and stepping continues."""
+@pytest.mark.parametrize("skin,inner", [(0.3, 0.0), (0.6, 0.2)])
+def test_list_autotune_keeps_the_state(gpu, skin, inner):
+    """The list-mode autotuner times rebuild, plain (and, with two-level lists, refresh) passes
+    at 1-3 bins per cut-off at the current positions: the state is not advanced, the forces it
+    leaves are those of a fresh context with the chosen bins (bit for bit), and stepping
+    continues."""
     p, box, col = _relaxed(8192, 4)
     md = HipMD(p, box, [sp["capacity"] for sp in col.species])
     md.upload(col)
-    md.set_neighbour_list(True, 0.3, 256)
+    md.set_neighbour_list(True, skin, 256, inner_skin=inner)
     md.forces(want=False)
     before = md.download(col.copy())
     best = md.autotune(reps=2)
@@ -401,7 +403,7 @@ def test_list_autotune_keeps_the_state(gpu):
     fresh = HipMD(p, box, [sp["capacity"] for sp in col.species])
     fresh.set_tuning(best["bins_per_cutoff"], 64)
     fresh.upload(before)
-    fresh.set_neighbour_list(True, 0.3, 256)
+    fresh.set_neighbour_list(True, skin, 256, inner_skin=inner)
     fresh.forces(want=False)
     ref = fresh.download(col.copy())
     fresh.close()
