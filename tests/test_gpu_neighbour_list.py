@@ -251,6 +251,30 @@ def test_two_level_lists_bit_identical_to_one_level(gpu, inner):
     assert np.array_equal(f1, f2)
 
 
+def test_two_level_lists_2d_two_disc_cells(gpu):
+    """The same bit-identity in 2D with two-disc cells (harmonic bond, motility, soft core):
+    2,000 cells, 3,000 steps of dt 1e-3, one-level lists of skin 0.4 against two-level lists of
+    0.4 / 0.1; discs move beyond half the outer skin, so both lists are rebuilt on the way."""
+    p, box, col = configs.plain_2d_sc(n_cells=2000, L=64.0, seed=3, dt=1e-3, precision=abi.PREC_FP32)
+    out = []
+    for inner in (0.0, 0.1):
+        md = HipMD(p, box, [sp["capacity"] for sp in col.species])
+        md.upload(col)
+        md.forces(want=False)
+        md.set_neighbour_list(True, 0.4, 256, inner_skin=inner)
+        assert md.step(3000, 0) == 3000
+        c = col.copy()
+        md.download(c)
+        md.close()
+        out.append(c.flat("pos").reshape(-1, 2))
+    d = out[0] - col.flat("pos").reshape(-1, 2)
+    d -= 64.0 * np.round(d / 64.0)
+    move = float(np.sqrt((d * d).sum(1)).max())
+    say(f"two-level lists, 2D two-disc cells: largest move {move:.3f} sigma; identical {np.array_equal(out[0], out[1])}")
+    assert move > 0.2
+    assert np.array_equal(out[0], out[1])
+
+
 def test_two_level_lists_reject_unsupported_setups(gpu):
     p, box, col = configs.relaxed_bulk_3d(4096, seed=6, precision=abi.PREC_FP32)
     md = HipMD(p, box, [sp["capacity"] for sp in col.species])
